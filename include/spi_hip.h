@@ -1,0 +1,160 @@
+/*
+ * spi_hip.h — C ABI of the MI355X (gfx950) engine for SPI-Active's sysID hot path:
+ * batched Go2 rollouts under sampled physical parameters, scored by a tracking cost and
+ * driven by a CEM update.
+ *
+ * Plain pointers and sizes only.  Every pointer argument named *_dev is device memory
+ * (e.g. a torch tensor's data_ptr()); every entry point takes the hipStream_t it is
+ * enqueued on (pass torch.cuda.current_stream().cuda_stream) and never synchronises it
+ * unless its name ends in _sync.  Return value: SPI_OK or an SpiStatus error code;
+ * spi_last_error() describes the last failure of the calling thread.
+ *
+ * Parameter vectors are SoA: params_dev[p * ld + i] is parameter p of env/candidate i,
+ * p in [0, SPI_P_COUNT).  Root states use IsaacGym's layout (isaacgym.py:567-572):
+ * [pos(3), quat xyzw(4), linvel(3), angvel(3)] in world axes, linvel = velocity of the
+ * base frame origin.  DOF state is [n][12][2] = (q, qd) as isaacgym.py:574-576.
+ */
+#ifndef SPI_HIP_H
+#define SPI_HIP_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPI_ABI_VERSION 1
+#define SPI_NUM_DOF 12
+#define SPI_ROOT_DIM 13
+#define SPI_STATE_DIM 37 /* root(13) + q(12) + qd(12) */
+#define SPI_NUM_FEET 4
+#define SPI_NUM_ISAAC_BODIES 19
+#define SPI_P_COUNT 25
+#define SPI_MAX_CEM_SAMPLES 8192
+
+typedef enum SpiStatus {
+  SPI_OK = 0,
+  SPI_ERR_INVALID_ARG = 1,
+  SPI_ERR_HIP = 2,
+  SPI_ERR_TOO_LARGE = 3
+} SpiStatus;
+
+typedef enum SpiMotorModel {
+  SPI_MOTOR_NONE = 0,
+  SPI_MOTOR_SCALAR = 1,    /* act2tau_scalar,    active_sysid_openloop.py:356-363 */
+  SPI_MOTOR_VEC3 = 2,      /* act2tau_vec3,      active_sysid_openloop.py:365-379 */
+  SPI_MOTOR_VEC3_TANH = 3  /* act2tau_vec3_tanh, active_sysid_openloop.py:381-400 */
+} SpiMotorModel;
+
+typedef enum SpiCostMode {
+  SPI_COST_TRACKING = 0,   /* sum over steps of weighted squared errors of all 6 groups */
+  SPI_COST_REFERENCE = 1   /* scripts/eval.py:287-292: horizon end, L2 norms of pos, quat, q */
+} SpiCostMode;
+
+/* Simulation settings shared by every lane of a launch. */
+typedef struct SpiSimArgs {
+  float dt;            /* physics step: 1 / sim.fps (isaacgym_active_sysid.yaml:15) */
+  int decimation;      /* physics steps per control step (isaacgym_active_sysid.yaml:16) */
+  int substeps;        /* integration sub-intervals per physics step, torque held */
+  float gravity;       /* 9.81 along -z (isaacgym.py:71) */
+  float kp[12];        /* PD stiffness per DOF (go2.yaml:78-90) */
+  float kd[12];        /* PD damping per DOF (go2.yaml:92-104) */
+  float q0[12];        /* default joint angles (go2.yaml:56-68) */
+  float effort[12];    /* torque clip (go2.yaml:39) */
+  float action_scale;  /* go2.yaml:107 */
+  float action_clip;   /* go2.yaml:108 */
+  float hip_scale;     /* 0.5 for go2_omni (go2_omni.py:437), 1.0 for legged_base */
+  int motor_model;     /* SpiMotorModel */
+  float k_contact;     /* penalty stiffness [N/m] */
+  float c_contact;     /* normal damping [N s/m] */
+  float mu;            /* Coulomb cone */
+  float c_tangent;     /* tangential viscous slope below the cone [N s/m] */
+  float fric_vel_inv;  /* 1 / velocity scale of the tanh joint friction [s/rad] */
+  float state_limit;   /* a lane whose |v|,|w|,|qd| sum exceeds this is marked diverged */
+} SpiSimArgs;
+
+typedef struct SpiCostArgs {
+  int mode;            /* SpiCostMode */
+  float w[6];          /* weights: pos, quat, linvel, angvel, q, qd (reference mode uses 0, 1, 4) */
+} SpiCostArgs;
+
+const char* spi_last_error(void);
+int spi_abi_version(void);
+
+/* Number of floats of one segment record: init state (37), then per control step the
+ * action (12) and the target state after that step (37). */
+int spi_segment_stride(int horizon);
+
+/* ---- BaseSimulator surface (replaces the IsaacGym calls behind spigym/simulator) ---- */
+
+/* One physics step for n envs in place: replaces gym.set_dof_actuation_force_tensor +
+ * gym.simulate + refresh_dof_state_tensor (isaacgym.py:598-599, 622-626).
+ * torques_dev [n][12]; foot_force_dev [n][4][3] world-frame contact forces or NULL. */
+int spi_physics_step(const SpiSimArgs* sim, const float* params_dev, int ld, int n,
+                     float* root_dev, float* dof_state_dev, const float* torques_dev,
+                     float* foot_force_dev, hipStream_t stream);
+
+/* Row scatter dst[idx[i]] = src[idx[i]] of row_len floats: replaces
+ * gym.set_actor_root_state_tensor_indexed (isaacgym.py:601-607, row_len 13) and
+ * gym.set_dof_state_tensor_indexed (isaacgym.py:615-620, row_len 24). */
+int spi_set_rows_indexed(float* dst_dev, const float* src_dev, const int* idx_dev, int n_idx, int n_rows,
+                         int row_len, hipStream_t stream);
+
+/* Rigid-body states [n][19][13] (pos, quat xyzw, linvel, angvel; world) of the IsaacGym
+ * body list (go2.yaml:44): replaces gym.refresh_rigid_body_state_tensor (isaacgym.py:582). */
+int spi_rigid_body_states(const float* root_dev, const float* dof_state_dev, int n, float* out_dev,
+                          hipStream_t stream);
+
+/* PD + clip + motor model for n envs: the torque computation of
+ * legged_robot_base.py:529-560 / go2_omni.py:423-465 + active_sysid_openloop.py:174-187.
+ * actions_dev [n][12] (raw, clipped inside), torques_dev [n][12]. */
+int spi_compute_torques(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* actions_dev,
+                        const float* dof_state_dev, float* torques_dev, hipStream_t stream);
+
+/* Finite-difference Fisher-information trace of active_sysid_openloop.py:402-426.
+ * Envs are grouped (1 main + n_param aux); out_dev[n] gets the group's ||X||_F^2 repeated
+ * for every env of the group (repeat_interleave). */
+int spi_fd_fim_trace(const float* root_dev, const float* dof_state_dev, const float* env_origins_dev,
+                     int n_main, int n_param, float delta, float* out_dev, hipStream_t stream);
+
+/* ---- fused hot path ---- */
+
+/* Fused rollout + tracking cost: for every (segment s, candidate c) run `horizon` control
+ * steps of PD + motor model + ABA + contact + semi-implicit Euler from the segment's initial
+ * state and write cost_seg_dev[s * n_cand + c].  Replaces the eager loop of
+ * scripts/eval.py:239-296 (and the per-candidate Python loop of mass_landscape.py:123-126).
+ * seg_dev: [n_seg][spi_segment_stride(horizon)] floats.  Diverged lanes get +inf. */
+int spi_rollout_cost(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld, int n_cand,
+                     const float* seg_dev, int n_seg, int horizon, float* cost_seg_dev, hipStream_t stream);
+
+/* cost_dev[c] = sum_s seg_weight_dev[s] * cost_seg_dev[s * n_cand + c], fixed summation order. */
+int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, int n_seg, int n_cand,
+                        float* cost_dev, hipStream_t stream);
+
+/* Record mode (synthetic ground truth / replay): n lanes each with their own params,
+ * initial state init_dev [n][37] and actions actions_dev [n][T][12]; writes the state after
+ * every control step to states_dev [n][T][37]. */
+int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* init_dev,
+                 const float* actions_dev, int n_steps, float* states_dev, hipStream_t stream);
+
+/* ---- CEM (replaces the Optuna sampler of algo/active_sysid.yaml:20 / mass_opt.py:216) ---- */
+
+/* theta_dev[p * ld + k] = clamp(mean[p] + std[p] * z(seed, iter, p * n + k), lo[p], hi[p]),
+ * z standard normal from Philox4x32-10 + Box-Muller.  mean/std/lo/hi: [n_param] device. */
+int spi_cem_sample(const float* mean_dev, const float* std_dev, const float* lo_dev, const float* hi_dev, int n_param,
+                   int n, unsigned long long seed, unsigned int iteration, float* theta_dev, int ld,
+                   hipStream_t stream);
+
+/* Elite selection + update: sorts (cost, index) of n <= SPI_MAX_CEM_SAMPLES samples in LDS
+ * (bitonic), takes the n_elite lowest, writes new mean/std (std floored at min_std_dev[p],
+ * blended with the old moments by alpha: new = alpha*old + (1-alpha)*elite), the best sample
+ * and its cost (best_dev [n_param], best_cost_dev [1]: kept if not improved), and
+ * elite_idx_dev [n_elite] (may be NULL). */
+int spi_cem_update(const float* cost_dev, const float* theta_dev, int ld, int n_param, int n, int n_elite,
+                   float alpha, const float* min_std_dev, float* mean_dev, float* std_dev, float* best_dev,
+                   float* best_cost_dev, int* elite_idx_dev, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPI_HIP_H */

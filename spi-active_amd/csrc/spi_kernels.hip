@@ -1,0 +1,419 @@
+// HIP kernels and the C ABI (include/spi_hip.h) of the gfx950 sysID engine.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+#include "spi_dyn.h"
+
+using namespace spi;
+
+namespace {
+thread_local char g_err[512] = "";
+}  // namespace
+
+void spi_set_error(const char* msg) { std::snprintf(g_err, sizeof(g_err), "%s", msg); }
+
+namespace {
+
+int fail(int code, const char* msg) {
+  spi_set_error(msg);
+  return code;
+}
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return SPI_OK;
+  std::snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+  return SPI_ERR_HIP;
+}
+int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
+
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  Remap the
+// hardware block id so that logically consecutive blocks (same segment, neighbouring
+// candidate blocks) run on the same XCD and share its L2.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  constexpr int NX = 8;
+  int q = nb / NX, r = nb % NX;
+  int x = b % NX, i = b / NX;
+  int base = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
+  return base + i;
+}
+
+__device__ __forceinline__ LaneParams load_params(const float* __restrict__ par, int ld, int i) {
+  LaneParams lp;
+  lp.base = make_base_inertia(par[0 * ld + i], par[1 * ld + i], par[2 * ld + i], par[3 * ld + i], par[4 * ld + i],
+                              par[5 * ld + i], par[6 * ld + i], par[7 * ld + i], par[8 * ld + i], par[9 * ld + i]);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    lp.kp_s[l] = par[(10 + l) * ld + i];
+    lp.kd_s[l] = par[(14 + l) * ld + i];
+    lp.fric[l] = par[(18 + l) * ld + i];
+  }
+#pragma unroll
+  for (int g = 0; g < 3; ++g) lp.motor[g] = par[(22 + g) * ld + i];
+  return lp;
+}
+
+__device__ __forceinline__ float state_norm(const State& st) {
+  float s = fabsf(st.w.x) + fabsf(st.w.y) + fabsf(st.w.z) + fabsf(st.v.x) + fabsf(st.v.y) + fabsf(st.v.z);
+#pragma unroll
+  for (int j = 0; j < 12; ++j) s += fabsf(st.qd[j]);
+  return s;
+}
+
+// Weighted error of the state after a control step against a 37-float target record.
+__device__ __forceinline__ float step_cost(const SpiCostArgs& c, const State& st, const float* tg) {
+  G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
+  V3 vw = mul(R, st.v), ww = mul(R, st.w);
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f, e4 = 0.f, e5 = 0.f, d;
+  d = st.p.x - tg[0]; e0 += d * d;
+  d = st.p.y - tg[1]; e0 += d * d;
+  d = st.p.z - tg[2]; e0 += d * d;
+  d = st.qx - tg[3]; e1 += d * d;
+  d = st.qy - tg[4]; e1 += d * d;
+  d = st.qz - tg[5]; e1 += d * d;
+  d = st.qw - tg[6]; e1 += d * d;
+  d = vw.x - tg[7]; e2 += d * d;
+  d = vw.y - tg[8]; e2 += d * d;
+  d = vw.z - tg[9]; e2 += d * d;
+  d = ww.x - tg[10]; e3 += d * d;
+  d = ww.y - tg[11]; e3 += d * d;
+  d = ww.z - tg[12]; e3 += d * d;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    d = st.q[j] - tg[13 + j]; e4 += d * d;
+    d = st.qd[j] - tg[25 + j]; e5 += d * d;
+  }
+  if (c.mode == SPI_COST_TRACKING)
+    return c.w[0] * e0 + c.w[1] * e1 + c.w[2] * e2 + c.w[3] * e3 + c.w[4] * e4 + c.w[5] * e5;
+  return c.w[0] * sqrtf(e0) + c.w[1] * sqrtf(e1) + c.w[4] * sqrtf(e4);
+}
+
+constexpr int kRolloutThreads = 256;
+
+// One workgroup = one segment x 256 candidates.  The segment record is staged once in LDS
+// and read by every lane (uniform address: LDS broadcast); parameters are read coalesced
+// from the SoA array.
+__global__ __launch_bounds__(kRolloutThreads) void rollout_cost_kernel(
+    SpiSimArgs a, SpiCostArgs cost, const float* __restrict__ par, int ld, int n_cand,
+    const float* __restrict__ seg, int H, int stride, int n_cblk, float* __restrict__ out) {
+  extern __shared__ float lds[];
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = logical / n_cblk, cb = logical % n_cblk;
+  const float* src = seg + static_cast<size_t>(s) * stride;
+  for (int i = threadIdx.x; i < stride; i += blockDim.x) lds[i] = src[i];
+  __syncthreads();
+  const int c = cb * kRolloutThreads + threadIdx.x;
+  if (c >= n_cand) return;
+  const LaneParams lp = load_params(par, ld, c);
+  State st;
+  state_from_root(st, lds, lds + 13, lds + 25, 1);
+  float acc = 0.0f;
+  bool ok = true;
+  for (int t = 0; t < H; ++t) {
+    const float* rec = lds + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+    float act[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) act[j] = rec[j];
+    for (int d = 0; d < a.decimation; ++d) {
+      float tau[12];
+      control_torques(a, lp, act, st, tau);
+      physics_step(a, lp, st, tau, nullptr);
+    }
+    if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost(cost, st, rec + 12);
+    if (!(state_norm(st) < a.state_limit)) { ok = false; break; }
+  }
+  out[static_cast<size_t>(s) * n_cand + c] = (ok && isfinite(acc)) ? acc : INFINITY;
+}
+
+__global__ void reduce_segments_kernel(const float* __restrict__ cs, const float* __restrict__ w, int n_seg,
+                                       int n_cand, float* __restrict__ out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  float acc = 0.0f;
+  for (int s = 0; s < n_seg; ++s) {
+    float ws = w[s];
+    if (ws != 0.0f) acc += ws * cs[static_cast<size_t>(s) * n_cand + c];
+  }
+  out[c] = acc;
+}
+
+__global__ __launch_bounds__(256) void physics_step_kernel(SpiSimArgs a, const float* __restrict__ par, int ld,
+                                                           int n, float* __restrict__ root, float* __restrict__ dof,
+                                                           const float* __restrict__ torques,
+                                                           float* __restrict__ foot_force) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const LaneParams lp = load_params(par, ld, i);
+  State st;
+  float* r = root + static_cast<size_t>(i) * 13;
+  float* d = dof + static_cast<size_t>(i) * 24;
+  state_from_root(st, r, d, d + 1, 2);
+  float tau[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) tau[j] = torques[static_cast<size_t>(i) * 12 + j];
+  float ff[12];
+  physics_step(a, lp, st, tau, foot_force ? ff : nullptr);
+  state_to_root(st, r, d, d + 1, 2);
+  if (foot_force) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) foot_force[static_cast<size_t>(i) * 12 + k] = ff[k];
+  }
+}
+
+__global__ void compute_torques_kernel(SpiSimArgs a, const float* __restrict__ par, int ld, int n,
+                                       const float* __restrict__ actions, const float* __restrict__ dof,
+                                       float* __restrict__ torques) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const LaneParams lp = load_params(par, ld, i);
+  State st;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    st.q[j] = dof[static_cast<size_t>(i) * 24 + 2 * j];
+    st.qd[j] = dof[static_cast<size_t>(i) * 24 + 2 * j + 1];
+  }
+  float act[12], tau[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) act[j] = actions[static_cast<size_t>(i) * 12 + j];
+  control_torques(a, lp, act, st, tau);
+#pragma unroll
+  for (int j = 0; j < 12; ++j) torques[static_cast<size_t>(i) * 12 + j] = tau[j];
+}
+
+__global__ void set_rows_kernel(float* __restrict__ dst, const float* __restrict__ src, const int* __restrict__ idx,
+                                int n_idx, int n_rows, int row_len) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_idx * row_len) return;
+  int k = t / row_len, e = t % row_len;
+  int r = idx[k];
+  if (r < 0 || r >= n_rows) return;
+  dst[static_cast<size_t>(r) * row_len + e] = src[static_cast<size_t>(r) * row_len + e];
+}
+
+// Forward kinematics of the 19 IsaacGym bodies (rigid-body state tensor rows).
+__global__ void rigid_body_kernel(const float* __restrict__ root, const float* __restrict__ dof, int n,
+                                  float* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  State st;
+  state_from_root(st, root + static_cast<size_t>(i) * 13, dof + static_cast<size_t>(i) * 24,
+                  dof + static_cast<size_t>(i) * 24 + 1, 2);
+  G3 R0 = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
+  // body poses (world) and spatial velocities (body coords) of the 13 folded bodies
+  G3 Rb[13];
+  V3 pb[13];
+  M6 vb[13];
+  Rb[0] = R0;
+  pb[0] = st.p;
+  vb[0] = M6{st.w, st.v};
+#pragma unroll
+  for (int b = 1; b < 13; ++b) {
+    const int j = b - 1, parent = (j % 3 == 0) ? 0 : b - 1, K = SPI_JOINT_AXIS[j];
+    V3 r{float(SPI_JOINT_ORIGIN[3 * j]), float(SPI_JOINT_ORIGIN[3 * j + 1]), float(SPI_JOINT_ORIGIN[3 * j + 2])};
+    float s, c;
+    spi_sincos(st.q[j], &s, &c);
+    M6 vp = vb[parent];
+    M6 v;
+    if (K == 0) v = xm<0>(vp, r, c, s);
+    else if (K == 1) v = xm<1>(vp, r, c, s);
+    else v = xm<2>(vp, r, c, s);
+    if (K == 0) v.w.x += st.qd[j]; else if (K == 1) v.w.y += st.qd[j]; else v.w.z += st.qd[j];
+    vb[b] = v;
+    const G3& Rp = Rb[parent];
+    pb[b] = pb[parent] + mul(Rp, r);
+    // R_b = R_p * R_K(q): rotate the rows of R_p by R_K^T
+    V3 x0, x1, x2;
+    if (K == 0) { x0 = rotT<0>(V3{Rp.m00, Rp.m01, Rp.m02}, c, s); x1 = rotT<0>(V3{Rp.m10, Rp.m11, Rp.m12}, c, s); x2 = rotT<0>(V3{Rp.m20, Rp.m21, Rp.m22}, c, s); }
+    else if (K == 1) { x0 = rotT<1>(V3{Rp.m00, Rp.m01, Rp.m02}, c, s); x1 = rotT<1>(V3{Rp.m10, Rp.m11, Rp.m12}, c, s); x2 = rotT<1>(V3{Rp.m20, Rp.m21, Rp.m22}, c, s); }
+    else { x0 = rotT<2>(V3{Rp.m00, Rp.m01, Rp.m02}, c, s); x1 = rotT<2>(V3{Rp.m10, Rp.m11, Rp.m12}, c, s); x2 = rotT<2>(V3{Rp.m20, Rp.m21, Rp.m22}, c, s); }
+    Rb[b] = G3{x0.x, x0.y, x0.z, x1.x, x1.y, x1.z, x2.x, x2.y, x2.z};
+  }
+  float* o = out + static_cast<size_t>(i) * SPI_NISAAC * 13;
+#pragma unroll
+  for (int k = 0; k < SPI_NISAAC; ++k) {
+    const int b = SPI_ISAAC_BODY[k];
+    V3 off{float(SPI_ISAAC_OFFSET[3 * k]), float(SPI_ISAAC_OFFSET[3 * k + 1]), float(SPI_ISAAC_OFFSET[3 * k + 2])};
+    const G3& R = Rb[b];
+    V3 p = pb[b] + mul(R, off);
+    V3 vl = vb[b].v + cross(vb[b].w, off);
+    V3 vw = mul(R, vl), ww = mul(R, vb[b].w);
+    // rotation matrix -> quaternion (xyzw), Shepperd's method
+    float tr = R.m00 + R.m11 + R.m22, qx, qy, qz, qw;
+    if (tr > 0.f) {
+      float S = sqrtf(tr + 1.f) * 2.f;
+      qw = 0.25f * S; qx = (R.m21 - R.m12) / S; qy = (R.m02 - R.m20) / S; qz = (R.m10 - R.m01) / S;
+    } else if (R.m00 > R.m11 && R.m00 > R.m22) {
+      float S = sqrtf(1.f + R.m00 - R.m11 - R.m22) * 2.f;
+      qw = (R.m21 - R.m12) / S; qx = 0.25f * S; qy = (R.m01 + R.m10) / S; qz = (R.m02 + R.m20) / S;
+    } else if (R.m11 > R.m22) {
+      float S = sqrtf(1.f + R.m11 - R.m00 - R.m22) * 2.f;
+      qw = (R.m02 - R.m20) / S; qx = (R.m01 + R.m10) / S; qy = 0.25f * S; qz = (R.m12 + R.m21) / S;
+    } else {
+      float S = sqrtf(1.f + R.m22 - R.m00 - R.m11) * 2.f;
+      qw = (R.m10 - R.m01) / S; qx = (R.m02 + R.m20) / S; qy = (R.m12 + R.m21) / S; qz = 0.25f * S;
+    }
+    if (qw < 0.f) { qx = -qx; qy = -qy; qz = -qz; qw = -qw; }
+    float vals[13] = {p.x, p.y, p.z, qx, qy, qz, qw, vw.x, vw.y, vw.z, ww.x, ww.y, ww.z};
+#pragma unroll
+    for (int e = 0; e < 13; ++e) o[k * 13 + e] = vals[e];
+  }
+}
+
+// Finite-difference FIM trace, active_sysid_openloop.py:402-426: per group of (1 main + P aux)
+// envs, X = [(root_main - root_aux)/delta with origins removed from positions,
+// (q_main - q_aux)/delta], value = trace(X^T X) = ||X||_F^2, repeated for the group.
+__global__ void fd_fim_kernel(const float* __restrict__ root, const float* __restrict__ dof,
+                              const float* __restrict__ origins, int n_main, int P, float delta,
+                              float* __restrict__ out) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_main) return;
+  const int g = P + 1, im = m * g;
+  float acc = 0.0f;
+  for (int k = 1; k <= P; ++k) {
+    const int ia = im + k;
+    for (int e = 0; e < 13; ++e) {
+      float xm = root[static_cast<size_t>(im) * 13 + e], xa = root[static_cast<size_t>(ia) * 13 + e];
+      if (e < 3) { xm -= origins[static_cast<size_t>(im) * 3 + e]; xa -= origins[static_cast<size_t>(ia) * 3 + e]; }
+      float d = (xm - xa) / delta;
+      acc += d * d;
+    }
+    for (int j = 0; j < 12; ++j) {
+      float d = (dof[static_cast<size_t>(im) * 24 + 2 * j] - dof[static_cast<size_t>(ia) * 24 + 2 * j]) / delta;
+      acc += d * d;
+    }
+  }
+  for (int k = 0; k < g; ++k) out[im + k] = acc;
+}
+
+// Record mode: per-lane params, initial state and action sequence; state after every control step.
+__global__ __launch_bounds__(256) void simulate_kernel(SpiSimArgs a, const float* __restrict__ par, int ld, int n,
+                                                       const float* __restrict__ init,
+                                                       const float* __restrict__ actions, int T,
+                                                       float* __restrict__ states) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const LaneParams lp = load_params(par, ld, i);
+  State st;
+  const float* s0 = init + static_cast<size_t>(i) * SPI_STATE_DIM;
+  state_from_root(st, s0, s0 + 13, s0 + 25, 1);
+  for (int t = 0; t < T; ++t) {
+    float act[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) act[j] = actions[(static_cast<size_t>(i) * T + t) * 12 + j];
+    for (int d = 0; d < a.decimation; ++d) {
+      float tau[12];
+      control_torques(a, lp, act, st, tau);
+      physics_step(a, lp, st, tau, nullptr);
+    }
+    float* o = states + (static_cast<size_t>(i) * T + t) * SPI_STATE_DIM;
+    state_to_root(st, o, o + 13, o + 25, 1);
+  }
+}
+
+bool sim_args_ok(const SpiSimArgs* a) {
+  return a && a->dt > 0.f && a->decimation >= 1 && a->decimation <= 64 && a->substeps >= 1 && a->substeps <= 64 &&
+         a->motor_model >= 0 && a->motor_model <= 3 && a->state_limit > 0.f;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spi_last_error(void) { return g_err; }
+int spi_abi_version(void) { return SPI_ABI_VERSION; }
+int spi_segment_stride(int horizon) { return SPI_STATE_DIM + horizon * (12 + SPI_STATE_DIM); }
+
+int spi_physics_step(const SpiSimArgs* sim, const float* params_dev, int ld, int n, float* root_dev,
+                     float* dof_state_dev, const float* torques_dev, float* foot_force_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_physics_step: invalid SpiSimArgs");
+  if (n < 0 || ld < n || !params_dev || !root_dev || !dof_state_dev || !torques_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_physics_step: bad sizes or null pointer");
+  if (n == 0) return SPI_OK;
+  physics_step_kernel<<<(n + 255) / 256, 256, 0, stream>>>(*sim, params_dev, ld, n, root_dev, dof_state_dev,
+                                                          torques_dev, foot_force_dev);
+  return launch_check("physics_step_kernel");
+}
+
+int spi_set_rows_indexed(float* dst_dev, const float* src_dev, const int* idx_dev, int n_idx, int n_rows, int row_len,
+                         hipStream_t stream) {
+  if (n_idx < 0 || n_rows < 0 || row_len <= 0 || !dst_dev || !src_dev || (n_idx > 0 && !idx_dev))
+    return fail(SPI_ERR_INVALID_ARG, "spi_set_rows_indexed: bad arguments");
+  if (n_idx == 0 || dst_dev == src_dev) return SPI_OK;
+  long long total = static_cast<long long>(n_idx) * row_len;
+  if (total > 0x7fffffffLL) return fail(SPI_ERR_TOO_LARGE, "spi_set_rows_indexed: too many elements");
+  set_rows_kernel<<<static_cast<int>((total + 255) / 256), 256, 0, stream>>>(dst_dev, src_dev, idx_dev, n_idx, n_rows,
+                                                                              row_len);
+  return launch_check("set_rows_kernel");
+}
+
+int spi_rigid_body_states(const float* root_dev, const float* dof_state_dev, int n, float* out_dev,
+                          hipStream_t stream) {
+  if (n < 0 || !root_dev || !dof_state_dev || !out_dev) return fail(SPI_ERR_INVALID_ARG, "spi_rigid_body_states");
+  if (n == 0) return SPI_OK;
+  rigid_body_kernel<<<(n + 127) / 128, 128, 0, stream>>>(root_dev, dof_state_dev, n, out_dev);
+  return launch_check("rigid_body_kernel");
+}
+
+int spi_compute_torques(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* actions_dev,
+                        const float* dof_state_dev, float* torques_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_compute_torques: invalid SpiSimArgs");
+  if (n < 0 || ld < n || !params_dev || !actions_dev || !dof_state_dev || !torques_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_compute_torques: bad sizes or null pointer");
+  if (n == 0) return SPI_OK;
+  compute_torques_kernel<<<(n + 255) / 256, 256, 0, stream>>>(*sim, params_dev, ld, n, actions_dev, dof_state_dev,
+                                                             torques_dev);
+  return launch_check("compute_torques_kernel");
+}
+
+int spi_fd_fim_trace(const float* root_dev, const float* dof_state_dev, const float* env_origins_dev, int n_main,
+                     int n_param, float delta, float* out_dev, hipStream_t stream) {
+  if (n_main < 0 || n_param < 1 || !(delta != 0.0f) || !root_dev || !dof_state_dev || !env_origins_dev || !out_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_fd_fim_trace: bad arguments");
+  if (n_main == 0) return SPI_OK;
+  fd_fim_kernel<<<(n_main + 127) / 128, 128, 0, stream>>>(root_dev, dof_state_dev, env_origins_dev, n_main, n_param,
+                                                          delta, out_dev);
+  return launch_check("fd_fim_kernel");
+}
+
+int spi_rollout_cost(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld, int n_cand,
+                     const float* seg_dev, int n_seg, int horizon, float* cost_seg_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim) || !cost || (cost->mode != SPI_COST_TRACKING && cost->mode != SPI_COST_REFERENCE))
+    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost: invalid SpiSimArgs/SpiCostArgs");
+  if (n_cand < 0 || n_seg < 0 || horizon < 1 || ld < n_cand || !params_dev || !seg_dev || !cost_seg_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost: bad sizes or null pointer");
+  if (n_cand == 0 || n_seg == 0) return SPI_OK;
+  const int stride = spi_segment_stride(horizon);
+  const size_t lds_bytes = static_cast<size_t>(stride) * sizeof(float);
+  if (lds_bytes > 64 * 1024) return fail(SPI_ERR_TOO_LARGE, "spi_rollout_cost: horizon too long for LDS staging");
+  const int n_cblk = (n_cand + kRolloutThreads - 1) / kRolloutThreads;
+  const long long nblk = static_cast<long long>(n_cblk) * n_seg;
+  if (nblk > 0x7fffffffLL || static_cast<long long>(n_seg) * n_cand > (1LL << 31))
+    return fail(SPI_ERR_TOO_LARGE, "spi_rollout_cost: grid too large");
+  rollout_cost_kernel<<<static_cast<int>(nblk), kRolloutThreads, lds_bytes, stream>>>(
+      *sim, *cost, params_dev, ld, n_cand, seg_dev, horizon, stride, n_cblk, cost_seg_dev);
+  return launch_check("rollout_cost_kernel");
+}
+
+int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, int n_seg, int n_cand,
+                        float* cost_dev, hipStream_t stream) {
+  if (n_seg < 0 || n_cand < 0 || !cost_seg_dev || !seg_weight_dev || !cost_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_reduce_segments: bad arguments");
+  if (n_cand == 0) return SPI_OK;
+  reduce_segments_kernel<<<(n_cand + 255) / 256, 256, 0, stream>>>(cost_seg_dev, seg_weight_dev, n_seg, n_cand,
+                                                                   cost_dev);
+  return launch_check("reduce_segments_kernel");
+}
+
+int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* init_dev,
+                 const float* actions_dev, int n_steps, float* states_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_simulate: invalid SpiSimArgs");
+  if (n < 0 || ld < n || n_steps < 0 || !params_dev || !init_dev || !actions_dev || !states_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_simulate: bad sizes or null pointer");
+  if (n == 0 || n_steps == 0) return SPI_OK;
+  simulate_kernel<<<(n + 255) / 256, 256, 0, stream>>>(*sim, params_dev, ld, n, init_dev, actions_dev, n_steps,
+                                                       states_dev);
+  return launch_check("simulate_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+"""ctypes binding of libspi_hip.so (the C ABI declared in include/spi_hip.h).
+
+torch provides device memory and the stream only: every pointer passed here is a
+``tensor.data_ptr()`` and every call is enqueued on ``torch.cuda.current_stream()``.
+There is no fallback: without the built library or a GPU the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+import torch  # noqa: F401  -- must be loaded first so the HIP runtime (soname libamdhip64.so.7) is shared
+
+from . import build
+from .params import SimParams
+
+SPI_P_COUNT = 25
+SPI_STATE_DIM = 37
+SPI_MAX_CEM_SAMPLES = 8192
+
+
+class SpiSimArgs(C.Structure):
+    _fields_ = [("dt", C.c_float), ("decimation", C.c_int), ("substeps", C.c_int), ("gravity", C.c_float),
+                ("kp", C.c_float * 12), ("kd", C.c_float * 12), ("q0", C.c_float * 12), ("effort", C.c_float * 12),
+                ("action_scale", C.c_float), ("action_clip", C.c_float), ("hip_scale", C.c_float),
+                ("motor_model", C.c_int), ("k_contact", C.c_float), ("c_contact", C.c_float), ("mu", C.c_float),
+                ("c_tangent", C.c_float), ("fric_vel_inv", C.c_float), ("state_limit", C.c_float)]
+
+
+class SpiCostArgs(C.Structure):
+    _fields_ = [("mode", C.c_int), ("w", C.c_float * 6)]
+
+
+def sim_args(sim: SimParams, state_limit: float = 1.0e4) -> SpiSimArgs:
+    a = SpiSimArgs()
+    a.dt = sim.dt
+    a.decimation = sim.decimation
+    a.substeps = sim.substeps
+    a.gravity = sim.gravity
+    a.kp[:] = [float(x) for x in sim.kp]
+    a.kd[:] = [float(x) for x in sim.kd]
+    a.q0[:] = [float(x) for x in sim.q0]
+    a.effort[:] = [float(x) for x in sim.effort]
+    a.action_scale = sim.action_scale
+    a.action_clip = sim.action_clip
+    a.hip_scale = 0.5 if sim.control_mode == 1 else 1.0
+    a.motor_model = sim.motor_model
+    a.k_contact = sim.k_contact
+    a.c_contact = sim.c_contact
+    a.mu = sim.mu
+    a.c_tangent = sim.c_tangent
+    a.fric_vel_inv = 1.0 / sim.fric_vel
+    a.state_limit = state_limit
+    return a
+
+
+def cost_args(mode: int, weights) -> SpiCostArgs:
+    c = SpiCostArgs()
+    c.mode = int(mode)
+    c.w[:] = [float(x) for x in weights]
+    return c
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load the library (building it in-tree if it is missing)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = build.ensure_built()
+    L = C.CDLL(path)
+    vp, ip, fp = C.c_void_p, C.c_int, C.c_float
+    sa, ca = C.POINTER(SpiSimArgs), C.POINTER(SpiCostArgs)
+    L.spi_last_error.restype = C.c_char_p
+    L.spi_abi_version.restype = ip
+    L.spi_segment_stride.argtypes = [ip]
+    L.spi_segment_stride.restype = ip
+    sigs = {
+        "spi_physics_step": [sa, vp, ip, ip, vp, vp, vp, vp, vp],
+        "spi_set_rows_indexed": [vp, vp, vp, ip, ip, ip, vp],
+        "spi_rigid_body_states": [vp, vp, ip, vp, vp],
+        "spi_compute_torques": [sa, vp, ip, ip, vp, vp, vp, vp],
+        "spi_fd_fim_trace": [vp, vp, vp, ip, ip, fp, vp, vp],
+        "spi_rollout_cost": [sa, ca, vp, ip, ip, vp, ip, ip, vp, vp],
+        "spi_reduce_segments": [vp, vp, ip, ip, vp, vp],
+        "spi_simulate": [sa, vp, ip, ip, vp, vp, ip, vp, vp],
+        "spi_cem_sample": [vp, vp, vp, vp, ip, ip, C.c_ulonglong, C.c_uint, vp, ip, vp],
+        "spi_cem_update": [vp, vp, ip, ip, ip, ip, fp, vp, vp, vp, vp, vp, vp, vp],
+    }
+    for name, args in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ip
+    if L.spi_abi_version() != 1:
+        raise RuntimeError("libspi_hip.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().spi_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("spi_amd needs a ROCm GPU: the engine has no CPU path")
+
+
+def ptr(t: Optional[torch.Tensor], dtype=torch.float32) -> Optional[int]:
+    """Device pointer of a contiguous GPU tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("tensor must live on the GPU")
+    if t.dtype != dtype:
+        raise TypeError(f"expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def stream() -> int:
+    return torch.cuda.current_stream().cuda_stream

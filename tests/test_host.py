@@ -1,0 +1,126 @@
+"""Host-side logic and the C ABI surface (CPU only: no device calls)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "spi_hip.h")
+
+
+def _header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(spi_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    import torch  # noqa: F401  (loads the HIP runtime first, as in production)
+    from spi_amd import build
+    path = build.ensure_built()
+    L = ctypes.CDLL(path)
+    names = _header_functions()
+    assert len(names) >= 13
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (spi_\w+)", out))
+    assert exported == set(names), exported ^ set(names)
+
+
+def test_ctypes_structs_match_c_layout():
+    """sizeof/offsetof of SpiSimArgs and SpiCostArgs as compiled by the host compiler."""
+    from spi_amd._lib import SpiCostArgs, SpiSimArgs
+    src = r"""
+    #include <cstdio>
+    #include <cstddef>
+    #include "spi_hip.h"
+    int main() {
+      std::printf("%zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs), offsetof(SpiSimArgs, effort),
+                  offsetof(SpiSimArgs, motor_model), offsetof(SpiSimArgs, state_limit), sizeof(SpiCostArgs),
+                  offsetof(SpiCostArgs, w));
+    }
+    """
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "l.cpp")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "l")
+        subprocess.run(["g++", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I",
+                        "/opt/rocm/include", c, "-o", exe], check=True)
+        got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    expect = [ctypes.sizeof(SpiSimArgs), SpiSimArgs.effort.offset, SpiSimArgs.motor_model.offset,
+              SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset]
+    assert got == expect
+
+
+def test_reference_segment_weights_known_answer():
+    """scripts/eval.py:279-310 on a hand-worked case: chunks of 4, motion ends at 2 and 6.
+    eval_mask per chunk = ~(cumsum(ends) > 0) -> [1,1,0,0]; total_valid = 6."""
+    from spi_amd.data import reference_segment_weights
+    ends = np.array([0, 0, 1, 0, 0, 0, 1, 0], bool)
+    w = reference_segment_weights(ends, 4)
+    assert np.allclose(w, np.array([1, 1, 0, 0, 1, 1, 0, 0]) / 6.0)
+    # one chunk holding everything: all windows after the first motion end are dropped
+    w2 = reference_segment_weights(ends, 8)
+    assert np.allclose(w2, np.array([1, 1, 0, 0, 0, 0, 0, 0]) / 6.0)
+
+
+def test_segment_packing_layout():
+    from spi_amd.data import SegmentSet, segment_stride
+    n, H = 3, 4
+    init = np.arange(n * 37, dtype=np.float32).reshape(n, 37)
+    acts = 1000 + np.arange(n * H * 12, dtype=np.float32).reshape(n, H, 12)
+    tg = 5000 + np.arange(n * H * 37, dtype=np.float32).reshape(n, H, 37)
+    p = SegmentSet(init, acts, tg, np.zeros(n, bool)).packed()
+    assert p.shape == (n, segment_stride(H)) == (n, 37 + H * 49)
+    for s in range(n):
+        assert np.array_equal(p[s, :37], init[s])
+        for t in range(H):
+            o = 37 + t * 49
+            assert np.array_equal(p[s, o:o + 12], acts[s, t])
+            assert np.array_equal(p[s, o + 12:o + 49], tg[s, t])
+
+
+def test_npz_windows_follow_eval_loader(tmp_path):
+    """scripts/eval.py:125-152: num_samples = T - H, init at t, target at t + H,
+    action window t..t+H-1, last window flagged, per-file gains."""
+    from spi_amd.data import load_npz_dataset, save_npz_trajectory
+    T, H = 12, 3
+    rng = np.random.default_rng(0)
+    files = []
+    for k, kp in enumerate((20.0, 25.0)):
+        st = rng.normal(size=(T, 37)).astype(np.float32)
+        ac = rng.normal(size=(T, 12)).astype(np.float32)
+        f = str(tmp_path / f"t{k}.npz")
+        save_npz_trajectory(f, st, ac, 0.02, kp, 0.5)
+        files.append((f, st, ac))
+    ds = load_npz_dataset([f for f, _, _ in files], H)
+    assert ds.n_seg == 2 * (T - H)
+    st0, ac0 = files[0][1], files[0][2]
+    assert np.array_equal(ds.init[4], st0[4])
+    assert np.array_equal(ds.targets[4, -1], st0[4 + H])
+    assert np.array_equal(ds.actions[4], ac0[4:4 + H])
+    assert ds.motion_ends.tolist() == ([False] * (T - H - 1) + [True]) * 2
+    assert ds.kp[0, 0] == 20.0 and ds.kp[-1, 0] == 25.0
+
+
+def test_parameter_layout(model):
+    from spi_amd.params import PARAM_NAMES, P_COUNT, nominal_params
+    assert P_COUNT == 25 and len(PARAM_NAMES) == 25
+    p = nominal_params(model)
+    assert p[0] == 6.921 and np.allclose(p[1:4], [0.021112, 0, -0.005366])
+    assert np.allclose(p[4:10], [0.02448, 0.098077, 0.107, 0.00012166, 0.0014849, -3.12e-05])
+    assert np.all(p[10:18] == 1.0)
+
+
+def test_sinusoid_actions_amplitude():
+    from spi_amd.data import sinusoid_actions
+    rng = np.random.default_rng(0)
+    a = sinusoid_actions(rng, 50, 500, 0.02, 0.25, amp=(0.1, 0.4), freq=(0.5, 3.0))
+    target = 0.25 * a                      # PD target offsets
+    peak = np.abs(target).max(axis=1)
+    assert peak.min() > 0.09 and peak.max() <= 0.4 + 1e-6
