@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark: 22-parameter CEM system identification of the Go2 on one MI355X.
+
+Workload (one "step" = one full sysID solve, inputs resident in HBM):
+  4096 parameter samples / iteration, 410 elites (ceil 10 %), 12 iterations,
+  256 synthetic segments of horizon 20 control steps (dt 0.005 s, decimation 4,
+  4 integration sub-intervals per physics step), PD Kp 20 / Kd 0.5, penalty contact
+  k 5e4 N/m, c 100 N s/m, mu 0.8.  Segments come from a seeded ground truth with
+  sinusoidal PD targets (amp U[0.1, 0.4] rad, f U[0.5, 3] Hz) and measurement noise
+  (sigma_q 0.01 rad, sigma_v 0.05 rad/s), generated on the GPU before timing.
+  Per solve: 4096 * 256 * 20 * 12 = 251,658,240 control steps.
+
+Prints one JSON line (metric, value = control steps per second over the timed solves).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spi-active_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "env-steps/s, sysID solves/s at ≤1% param error; p50/p99 solve latency"
+
+# Algorithmic FLOPs per control step of the fused rollout (PD + motor model + 4 x 4 ABA
+# sub-intervals + contact + integration + tracking cost), from the census program
+# spi-active_amd/csrc/spi_flop_census.cpp (counting scalar over the kernel's own code).
+FLOPS_PER_CONTROL_STEP = 83828.9
+PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 (vector) peak
+# HBM bytes per rollout_cost_kernel launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, scaled
+# by a copy-kernel calibration); None until measured for this workload.
+TRAFFIC_BYTES_PER_LAUNCH = None
+
+N_SAMPLES, N_SEG, HORIZON, ITERS = 4096, 256, 20, 12
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-candidates", type=int, default=64)
+    p.add_argument("--profile-launch", action="store_true",
+                   help="run one rollout_cost launch of the workload and a 256 MiB device copy (for rocprofv3 "
+                        "--pmc FETCH_SIZE / WRITE_SIZE calibration), then exit")
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(candidates x segments would shard across GPUs with one cost all-reduce per CEM iteration)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from spi_amd.agents.cem import CEMConfig, DeviceCEM
+    from spi_amd.agents.sysid import SysIdProblem, ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.model import load_model
+    from spi_amd.params import PARAM_NAMES, SimParams
+    from spi_amd.rollout import RolloutEngine
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    model = load_model()
+    sim = SimParams()                      # Kp 20, Kd 0.5, k 5e4, c 100, mu 0.8, dt 5 ms, decimation 4
+    prior = identification_prior()
+    theta_true = ground_truth(prior, seed=1)
+    eng = RolloutEngine(sim)
+    segs = synthetic_segments(eng.simulate_np, theta_true, sim, n_seg=N_SEG, horizon=HORIZON, seed=2)
+    seg_dev = eng.upload_segments(segs)
+    problem = SysIdProblem(eng, seg_dev, HORIZON, max_cand=N_SAMPLES)
+    stream = torch.cuda.current_stream()
+
+    if args.profile_launch:
+        cem = DeviceCEM(prior.mean, prior.std, prior.lo, prior.hi, prior.min_std, CEMConfig(N_SAMPLES, 0.1, ITERS))
+        theta = cem.ask()
+        eng.segment_costs(theta, N_SAMPLES, seg_dev, HORIZON, out=problem._seg_cost)
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)      # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_launch": "done", "copy_bytes": src.numel() * 4}))
+        return
+
+    kernel_events = []
+
+    def evaluate_timed(theta, n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        eng.segment_costs(theta, n, seg_dev, HORIZON, out=problem._seg_cost)
+        e1.record(stream)
+        kernel_events.append((e0, e1))
+        return eng.reduce(problem._seg_cost, problem.seg_weights, out=problem._cost[:n])
+
+    def solve(seed, timed):
+        cem = DeviceCEM(prior.mean, prior.std, prior.lo, prior.hi, prior.min_std,
+                        CEMConfig(N_SAMPLES, 0.1, ITERS, seed=seed), device="cuda")
+        cem.solve(evaluate_timed if timed else problem)
+        return cem
+
+    for w in range(args.warmup):
+        solve(1000 + w, timed=False)
+    torch.cuda.synchronize()
+
+    solve_events = []
+    t0 = time.perf_counter()
+    cems = []
+    for k in range(args.steps):
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record(stream)
+        cems.append(solve(k, timed=True))
+        s1.record(stream)
+        solve_events.append((s0, s1))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+
+    lat = np.array([a.elapsed_time(b) for a, b in solve_events])
+    kern = np.array([a.elapsed_time(b) for a, b in kernel_events])
+    ctrl_per_solve = N_SAMPLES * N_SEG * HORIZON * ITERS
+    value = args.steps * ctrl_per_solve / total
+
+    est = cems[-1].mean.cpu().numpy().astype(np.float64)
+    free = prior.free
+    rel = np.abs(est - theta_true) / np.maximum(np.abs(theta_true), 1e-12)
+    mass_err = float(rel[0])
+    well = [i for i in range(len(rel)) if free[i] and abs(theta_true[i]) > 1e-2]
+    param_err = {PARAM_NAMES[i]: round(float(rel[i]), 5) for i in range(len(rel)) if free[i]}
+
+    launch_ctrl = N_SAMPLES * N_SEG * HORIZON
+    kern_s = float(kern.mean()) / 1e3
+    achieved = FLOPS_PER_CONTROL_STEP * launch_ctrl / kern_s / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": TRAFFIC_BYTES_PER_LAUNCH,
+                "kernel": "rollout_cost_kernel", "kernel_ms_avg": round(float(kern.mean()), 3),
+                "flops_per_control_step": FLOPS_PER_CONTROL_STEP, "control_steps_per_launch": launch_ctrl}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(model, sim, segs, prior, args.cpu_candidates)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "cem22_go2", "samples": N_SAMPLES, "elites": CEMConfig(N_SAMPLES).n_elite,
+                   "iterations": ITERS, "segments": N_SEG, "horizon": HORIZON, "params": int(free.sum()),
+                   "dt": sim.dt, "decimation": sim.decimation, "substeps": sim.substeps,
+                   "control_steps_per_solve": ctrl_per_solve},
+        "solves_per_s": round(args.steps / total, 4),
+        "solve_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                             "p99": round(float(np.percentile(lat, 99)), 2)},
+        "param_error": {"base_mass_rel": round(mass_err, 5),
+                        "max_rel_nonzero_params": round(float(rel[well].max()), 5), "per_param": param_err},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(model, sim, segs, prior, n_cand):
+    """Oracle FP32 engine (OpenMP over the host cores) on a bounded sample of the workload."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import spio
+        spio.build()
+        cores = len(os.sched_getaffinity(0))
+        cores = min(cores, 16)
+        o = spio.Oracle(model, sim, f32=True)
+        rng = np.random.default_rng(0)
+        rows = np.repeat(prior.mean[None], n_cand, axis=0)
+        rows[:, prior.free] += rng.normal(size=(n_cand, int(prior.free.sum()))) * prior.std[prior.free]
+        rows = np.clip(rows, prior.lo, prior.hi).astype(np.float32)
+        w = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
+        t = time.perf_counter()
+        o.rollout_cost_f32(rows, segs.init, segs.actions, segs.targets, 0, w, n_threads=cores)
+        dt = time.perf_counter() - t
+        steps = n_cand * segs.n_seg * segs.horizon
+        return {"value": round(steps / dt, 1), "unit": "env-steps/s", "cores": cores, "kind": "port",
+                "sample": f"{n_cand} of {N_SAMPLES} CEM candidates x {segs.n_seg} segments x H={segs.horizon} "
+                          f"({steps} control steps, {dt:.1f} s), FP32 oracle engine on the unfolded URDF tree"}
+    except Exception as e:  # the baseline is reported, never fatal
+        return {"value": None, "unit": "env-steps/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
+
+
+if __name__ == "__main__":
+    main()

@@ -94,6 +94,14 @@ int spi_physics_step(const SpiSimArgs* sim, const float* params_dev, int ld, int
                      float* root_dev, float* dof_state_dev, const float* torques_dev,
                      float* foot_force_dev, hipStream_t stream);
 
+/* Forward dynamics without integration: acc_dev [n][18] = base spatial acceleration in base
+ * coordinates [angular(3), linear(3)] (gravity included) followed by qdd(12), for the motor
+ * torques torques_dev [n][12] (joint friction added inside).  The quantity PhysX computes inside
+ * gym.simulate (isaacgym.py:623); exposed for sensitivities and tests. */
+int spi_forward_dynamics(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* root_dev,
+                         const float* dof_state_dev, const float* torques_dev, float* acc_dev, float* foot_force_dev,
+                         hipStream_t stream);
+
 /* Row scatter dst[idx[i]] = src[idx[i]] of row_len floats: replaces
  * gym.set_actor_root_state_tensor_indexed (isaacgym.py:601-607, row_len 13) and
  * gym.set_dof_state_tensor_indexed (isaacgym.py:615-620, row_len 24). */
@@ -136,6 +144,21 @@ int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, 
  * every control step to states_dev [n][T][37]. */
 int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* init_dev,
                  const float* actions_dev, int n_steps, float* states_dev, hipStream_t stream);
+
+/* Active-sysID objective over open-loop excitations (active_sysid_openloop.py:116-131, 196-252,
+ * 316-330, 402-426; active_sysid.py:527-600): n_group groups of (1 main + n_param aux) envs,
+ * env g*(n_param+1)+k uses params column g*(n_param+1)+k (aux k = main + delta on one
+ * parameter), all envs of a group replay actions_dev [n_group][T][12] from their own initial
+ * state init_dev [n_env][37].  After every control step the group's FD-FIM trace
+ * ||X||_F^2 (X = (main - aux) / delta over root(13) + q(12)) is accumulated;
+ * fim_dev [n_group] receives the sum over the T steps.  Every ksync steps (at steps
+ * t % ksync == 1, counting from 1) the aux envs are synchronised to the main env:
+ * ksync_mode 0 = as the reference does it (only the first physics step's PD torque sees the
+ * main's joint state, active_sysid_openloop.py:283-290 writes views that are not pushed),
+ * 1 = full state copy.  ksync <= 0 disables.  n_param + 1 <= 64. */
+int spi_fim_rollout(const SpiSimArgs* sim, const float* params_dev, int ld, int n_group, int n_param,
+                    const float* init_dev, const float* actions_dev, int n_steps, float delta, int ksync,
+                    int ksync_mode, float* fim_dev, hipStream_t stream);
 
 /* ---- CEM (replaces the Optuna sampler of algo/active_sysid.yaml:20 / mass_opt.py:216) ---- */
 

@@ -70,6 +70,22 @@ __global__ void cem_sample_kernel(const float* __restrict__ mean, const float* _
   }
 }
 
+// cost_dev[c] = sum_s w[s] * cost_seg[s][c]: one thread per candidate, segments in a fixed
+// order, so the per-candidate cost (and the CEM ranking) is deterministic.  Built with strict
+// IEEE semantics so +inf (diverged) costs propagate.
+__global__ void reduce_segments_kernel(const float* __restrict__ cs, const float* __restrict__ w, int n_seg,
+                                       int n_cand, float* __restrict__ out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n_cand) return;
+  float acc = 0.0f;
+  for (int s = 0; s < n_seg; ++s) {
+    float ws = w[s];
+    if (ws != 0.0f) acc += ws * cs[static_cast<size_t>(s) * n_cand + c];
+  }
+  out[c] = acc;
+}
+
+
 constexpr int kCemThreads = 1024;
 
 // Single workgroup: (cost, index) pairs sorted ascending in LDS (bitonic network, ties broken
@@ -171,6 +187,18 @@ int spi_cem_update(const float* cost_dev, const float* theta_dev, int ld, int n_
   if (npow2 < 2) npow2 = 2;
   cem_update_kernel<<<1, kCemThreads, 0, stream>>>(cost_dev, theta_dev, ld, n_param, n, n_elite, alpha, min_std_dev,
                                                    mean_dev, std_dev, best_dev, best_cost_dev, elite_idx_dev, npow2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
+  return SPI_OK;
+}
+
+int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, int n_seg, int n_cand,
+                        float* cost_dev, hipStream_t stream) {
+  if (n_seg < 0 || n_cand < 0 || !cost_seg_dev || !seg_weight_dev || !cost_dev)
+    return cem_fail(SPI_ERR_INVALID_ARG, "spi_reduce_segments: bad arguments");
+  if (n_cand == 0) return SPI_OK;
+  reduce_segments_kernel<<<(n_cand + 255) / 256, 256, 0, stream>>>(cost_seg_dev, seg_weight_dev, n_seg, n_cand,
+                                                                   cost_dev);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
   return SPI_OK;

@@ -54,40 +54,9 @@ __device__ __forceinline__ LaneParams load_params(const float* __restrict__ par,
   return lp;
 }
 
-__device__ __forceinline__ float state_norm(const State& st) {
-  float s = fabsf(st.w.x) + fabsf(st.w.y) + fabsf(st.w.z) + fabsf(st.v.x) + fabsf(st.v.y) + fabsf(st.v.z);
-#pragma unroll
-  for (int j = 0; j < 12; ++j) s += fabsf(st.qd[j]);
-  return s;
-}
-
-// Weighted error of the state after a control step against a 37-float target record.
-__device__ __forceinline__ float step_cost(const SpiCostArgs& c, const State& st, const float* tg) {
-  G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
-  V3 vw = mul(R, st.v), ww = mul(R, st.w);
-  float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f, e4 = 0.f, e5 = 0.f, d;
-  d = st.p.x - tg[0]; e0 += d * d;
-  d = st.p.y - tg[1]; e0 += d * d;
-  d = st.p.z - tg[2]; e0 += d * d;
-  d = st.qx - tg[3]; e1 += d * d;
-  d = st.qy - tg[4]; e1 += d * d;
-  d = st.qz - tg[5]; e1 += d * d;
-  d = st.qw - tg[6]; e1 += d * d;
-  d = vw.x - tg[7]; e2 += d * d;
-  d = vw.y - tg[8]; e2 += d * d;
-  d = vw.z - tg[9]; e2 += d * d;
-  d = ww.x - tg[10]; e3 += d * d;
-  d = ww.y - tg[11]; e3 += d * d;
-  d = ww.z - tg[12]; e3 += d * d;
-#pragma unroll
-  for (int j = 0; j < 12; ++j) {
-    d = st.q[j] - tg[13 + j]; e4 += d * d;
-    d = st.qd[j] - tg[25 + j]; e5 += d * d;
-  }
-  if (c.mode == SPI_COST_TRACKING)
-    return c.w[0] * e0 + c.w[1] * e1 + c.w[2] * e2 + c.w[3] * e3 + c.w[4] * e4 + c.w[5] * e5;
-  return c.w[0] * sqrtf(e0) + c.w[1] * sqrtf(e1) + c.w[4] * sqrtf(e4);
-}
+// Finiteness from the exponent bits: stays correct when the file is compiled with
+// -ffinite-math-only (which lets the compiler fold the model's many x*0 terms).
+__device__ __forceinline__ bool finite_bits(float x) { return (__float_as_uint(x) & 0x7f800000u) != 0x7f800000u; }
 
 constexpr int kRolloutThreads = 256;
 
@@ -121,21 +90,12 @@ __global__ __launch_bounds__(kRolloutThreads) void rollout_cost_kernel(
       physics_step(a, lp, st, tau, nullptr);
     }
     if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost(cost, st, rec + 12);
-    if (!(state_norm(st) < a.state_limit)) { ok = false; break; }
+    const float nrm = state_norm(st);
+    if (!finite_bits(nrm) || nrm >= a.state_limit) { ok = false; break; }
   }
-  out[static_cast<size_t>(s) * n_cand + c] = (ok && isfinite(acc)) ? acc : INFINITY;
-}
-
-__global__ void reduce_segments_kernel(const float* __restrict__ cs, const float* __restrict__ w, int n_seg,
-                                       int n_cand, float* __restrict__ out) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n_cand) return;
-  float acc = 0.0f;
-  for (int s = 0; s < n_seg; ++s) {
-    float ws = w[s];
-    if (ws != 0.0f) acc += ws * cs[static_cast<size_t>(s) * n_cand + c];
-  }
-  out[c] = acc;
+  // diverged lanes get +inf, written as a bit pattern (this file is built with finite-math)
+  reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] =
+      (ok && finite_bits(acc)) ? __float_as_uint(acc) : 0x7f800000u;
 }
 
 __global__ __launch_bounds__(256) void physics_step_kernel(SpiSimArgs a, const float* __restrict__ par, int ld,
@@ -155,6 +115,32 @@ __global__ __launch_bounds__(256) void physics_step_kernel(SpiSimArgs a, const f
   float ff[12];
   physics_step(a, lp, st, tau, foot_force ? ff : nullptr);
   state_to_root(st, r, d, d + 1, 2);
+  if (foot_force) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) foot_force[static_cast<size_t>(i) * 12 + k] = ff[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void forward_dynamics_kernel(SpiSimArgs a, const float* __restrict__ par, int ld,
+                                                               int n, const float* __restrict__ root,
+                                                               const float* __restrict__ dof,
+                                                               const float* __restrict__ torques,
+                                                               float* __restrict__ acc, float* __restrict__ foot_force) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const LaneParams lp = load_params(par, ld, i);
+  State st;
+  state_from_root(st, root + static_cast<size_t>(i) * 13, dof + static_cast<size_t>(i) * 24,
+                  dof + static_cast<size_t>(i) * 24 + 1, 2);
+  float tau[12], qdd[12], ff[12];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) tau[j] = torques[static_cast<size_t>(i) * 12 + j];
+  M6 a0;
+  forward_dynamics(a, lp, st, tau, a0, qdd, foot_force ? ff : nullptr);
+  float* o = acc + static_cast<size_t>(i) * 18;
+  o[0] = a0.w.x; o[1] = a0.w.y; o[2] = a0.w.z; o[3] = a0.v.x; o[4] = a0.v.y; o[5] = a0.v.z;
+#pragma unroll
+  for (int j = 0; j < 12; ++j) o[6 + j] = qdd[j];
   if (foot_force) {
 #pragma unroll
     for (int k = 0; k < 12; ++k) foot_force[static_cast<size_t>(i) * 12 + k] = ff[k];
@@ -210,7 +196,7 @@ __global__ void rigid_body_kernel(const float* __restrict__ root, const float* _
 #pragma unroll
   for (int b = 1; b < 13; ++b) {
     const int j = b - 1, parent = (j % 3 == 0) ? 0 : b - 1, K = SPI_JOINT_AXIS[j];
-    V3 r{float(SPI_JOINT_ORIGIN[3 * j]), float(SPI_JOINT_ORIGIN[3 * j + 1]), float(SPI_JOINT_ORIGIN[3 * j + 2])};
+    CV3 r{float(SPI_JOINT_ORIGIN[3 * j]), float(SPI_JOINT_ORIGIN[3 * j + 1]), float(SPI_JOINT_ORIGIN[3 * j + 2])};
     float s, c;
     spi_sincos(st.q[j], &s, &c);
     M6 vp = vb[parent];
@@ -221,7 +207,7 @@ __global__ void rigid_body_kernel(const float* __restrict__ root, const float* _
     if (K == 0) v.w.x += st.qd[j]; else if (K == 1) v.w.y += st.qd[j]; else v.w.z += st.qd[j];
     vb[b] = v;
     const G3& Rp = Rb[parent];
-    pb[b] = pb[parent] + mul(Rp, r);
+    pb[b] = pb[parent] + mul(Rp, cvec(r));
     // R_b = R_p * R_K(q): rotate the rows of R_p by R_K^T
     V3 x0, x1, x2;
     if (K == 0) { x0 = rotT<0>(V3{Rp.m00, Rp.m01, Rp.m02}, c, s); x1 = rotT<0>(V3{Rp.m10, Rp.m11, Rp.m12}, c, s); x2 = rotT<0>(V3{Rp.m20, Rp.m21, Rp.m22}, c, s); }
@@ -311,6 +297,93 @@ __global__ __launch_bounds__(256) void simulate_kernel(SpiSimArgs a, const float
   }
 }
 
+// Active-sysID FD-FIM rollout.  Group of G = n_param + 1 envs on G2 = pow2 >= G consecutive
+// lanes of one wavefront (padding lanes shadow the main env and contribute nothing); states
+// of the main env are read by the aux lanes with cross-lane shuffles.
+__global__ __launch_bounds__(256) void fim_rollout_kernel(SpiSimArgs a, const float* __restrict__ par, int ld,
+                                                          int n_group, int P, int G2, const float* __restrict__ init,
+                                                          const float* __restrict__ actions, int T, float delta,
+                                                          int ksync, int ksync_mode, float* __restrict__ fim) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int group = gid / G2, member = gid % G2;
+  const bool active = group < n_group;
+  const int g = active ? group : n_group - 1;
+  const int m = member <= P ? member : 0;              // padding lanes shadow the main env
+  const int env = g * (P + 1) + m;
+  const int lane = threadIdx.x & 63;
+  const int main_lane = lane - member;                 // lane of this group's main env
+  const LaneParams lp = load_params(par, ld, env);
+  State st;
+  const float* s0 = init + static_cast<size_t>(env) * SPI_STATE_DIM;
+  state_from_root(st, s0, s0 + 13, s0 + 25, 1);
+  const float inv_d = 1.0f / delta;
+  float acc = 0.0f;
+  bool use_main_dof = false;
+  float mq[12], mqd[12];
+  for (int t = 0; t < T; ++t) {
+    float act[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) act[j] = actions[(static_cast<size_t>(g) * T + t) * 12 + j];
+    for (int d = 0; d < a.decimation; ++d) {
+      float tau[12];
+      if (use_main_dof && d == 0) {
+        // reference ksync: the synced joint state only reaches the first torque computation
+        State tmp = st;
+#pragma unroll
+        for (int j = 0; j < 12; ++j) { tmp.q[j] = mq[j]; tmp.qd[j] = mqd[j]; }
+        control_torques(a, lp, act, tmp, tau);
+      } else {
+        control_torques(a, lp, act, st, tau);
+      }
+      physics_step(a, lp, st, tau, nullptr);
+    }
+    use_main_dof = false;
+    // FD sensitivity against the main env: root in world frame (pos, quat, linvel, angvel) + q
+    G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
+    V3 vw = mul(R, st.v), ww = mul(R, st.w);
+    float x[25] = {st.p.x, st.p.y, st.p.z, st.qx, st.qy, st.qz, st.qw, vw.x, vw.y, vw.z, ww.x, ww.y, ww.z};
+#pragma unroll
+    for (int j = 0; j < 12; ++j) x[13 + j] = st.q[j];
+    float e = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 25; ++i) {
+      float xm = __shfl(x[i], main_lane, 64);
+      float dd = (xm - x[i]) * inv_d;
+      e += dd * dd;
+    }
+    if (member == 0 || member > P) e = 0.0f;
+    for (int off = 1; off < G2; off <<= 1) e += __shfl_xor(e, off, 64);
+    acc += e;
+    const int step = t + 1;
+    if (ksync > 0 && step % ksync == (ksync == 1 ? 0 : 1)) {
+      if (ksync_mode == 1) {
+        // full state copy from the main env
+        st.p.x = __shfl(st.p.x, main_lane, 64); st.p.y = __shfl(st.p.y, main_lane, 64);
+        st.p.z = __shfl(st.p.z, main_lane, 64);
+        st.qx = __shfl(st.qx, main_lane, 64); st.qy = __shfl(st.qy, main_lane, 64);
+        st.qz = __shfl(st.qz, main_lane, 64); st.qw = __shfl(st.qw, main_lane, 64);
+        st.w.x = __shfl(st.w.x, main_lane, 64); st.w.y = __shfl(st.w.y, main_lane, 64);
+        st.w.z = __shfl(st.w.z, main_lane, 64);
+        st.v.x = __shfl(st.v.x, main_lane, 64); st.v.y = __shfl(st.v.y, main_lane, 64);
+        st.v.z = __shfl(st.v.z, main_lane, 64);
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+          st.q[j] = __shfl(st.q[j], main_lane, 64);
+          st.qd[j] = __shfl(st.qd[j], main_lane, 64);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 12; ++j) {
+          mq[j] = __shfl(st.q[j], main_lane, 64);
+          mqd[j] = __shfl(st.qd[j], main_lane, 64);
+        }
+        use_main_dof = member != 0;
+      }
+    }
+  }
+  if (active && member == 0) fim[group] = acc;
+}
+
 bool sim_args_ok(const SpiSimArgs* a) {
   return a && a->dt > 0.f && a->decimation >= 1 && a->decimation <= 64 && a->substeps >= 1 && a->substeps <= 64 &&
          a->motor_model >= 0 && a->motor_model <= 3 && a->state_limit > 0.f;
@@ -333,6 +406,18 @@ int spi_physics_step(const SpiSimArgs* sim, const float* params_dev, int ld, int
   physics_step_kernel<<<(n + 255) / 256, 256, 0, stream>>>(*sim, params_dev, ld, n, root_dev, dof_state_dev,
                                                           torques_dev, foot_force_dev);
   return launch_check("physics_step_kernel");
+}
+
+int spi_forward_dynamics(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* root_dev,
+                         const float* dof_state_dev, const float* torques_dev, float* acc_dev, float* foot_force_dev,
+                         hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_forward_dynamics: invalid SpiSimArgs");
+  if (n < 0 || ld < n || !params_dev || !root_dev || !dof_state_dev || !torques_dev || !acc_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_forward_dynamics: bad sizes or null pointer");
+  if (n == 0) return SPI_OK;
+  forward_dynamics_kernel<<<(n + 255) / 256, 256, 0, stream>>>(*sim, params_dev, ld, n, root_dev, dof_state_dev,
+                                                              torques_dev, acc_dev, foot_force_dev);
+  return launch_check("forward_dynamics_kernel");
 }
 
 int spi_set_rows_indexed(float* dst_dev, const float* src_dev, const int* idx_dev, int n_idx, int n_rows, int row_len,
@@ -395,14 +480,25 @@ int spi_rollout_cost(const SpiSimArgs* sim, const SpiCostArgs* cost, const float
   return launch_check("rollout_cost_kernel");
 }
 
-int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, int n_seg, int n_cand,
-                        float* cost_dev, hipStream_t stream) {
-  if (n_seg < 0 || n_cand < 0 || !cost_seg_dev || !seg_weight_dev || !cost_dev)
-    return fail(SPI_ERR_INVALID_ARG, "spi_reduce_segments: bad arguments");
-  if (n_cand == 0) return SPI_OK;
-  reduce_segments_kernel<<<(n_cand + 255) / 256, 256, 0, stream>>>(cost_seg_dev, seg_weight_dev, n_seg, n_cand,
-                                                                   cost_dev);
-  return launch_check("reduce_segments_kernel");
+int spi_fim_rollout(const SpiSimArgs* sim, const float* params_dev, int ld, int n_group, int n_param,
+                    const float* init_dev, const float* actions_dev, int n_steps, float delta, int ksync,
+                    int ksync_mode, float* fim_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_fim_rollout: invalid SpiSimArgs");
+  if (n_group < 0 || n_param < 1 || n_param + 1 > 64 || n_steps < 1 || !(delta != 0.0f) ||
+      (ksync_mode != 0 && ksync_mode != 1) || !params_dev || !init_dev || !actions_dev || !fim_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_fim_rollout: bad arguments");
+  const long long n_env = static_cast<long long>(n_group) * (n_param + 1);
+  if (ld < n_env) return fail(SPI_ERR_INVALID_ARG, "spi_fim_rollout: ld < number of envs");
+  if (n_group == 0) return SPI_OK;
+  int G2 = 1;
+  while (G2 < n_param + 1) G2 <<= 1;
+  const long long lanes = static_cast<long long>(n_group) * G2;
+  if (lanes > 0x7fffffffLL) return fail(SPI_ERR_TOO_LARGE, "spi_fim_rollout: too many groups");
+  // blocks are whole wavefronts so groups never straddle a wavefront (G2 divides 64)
+  const int blocks = static_cast<int>((lanes + 255) / 256);
+  fim_rollout_kernel<<<blocks, 256, 0, stream>>>(*sim, params_dev, ld, n_group, n_param, G2, init_dev, actions_dev,
+                                                 n_steps, delta, ksync, ksync_mode, fim_dev);
+  return launch_check("fim_rollout_kernel");
 }
 
 int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* init_dev,

@@ -81,6 +81,7 @@ def lib() -> C.CDLL:
     L.spi_segment_stride.restype = ip
     sigs = {
         "spi_physics_step": [sa, vp, ip, ip, vp, vp, vp, vp, vp],
+        "spi_forward_dynamics": [sa, vp, ip, ip, vp, vp, vp, vp, vp, vp],
         "spi_set_rows_indexed": [vp, vp, vp, ip, ip, ip, vp],
         "spi_rigid_body_states": [vp, vp, ip, vp, vp],
         "spi_compute_torques": [sa, vp, ip, ip, vp, vp, vp, vp],
@@ -88,6 +89,7 @@ def lib() -> C.CDLL:
         "spi_rollout_cost": [sa, ca, vp, ip, ip, vp, ip, ip, vp, vp],
         "spi_reduce_segments": [vp, vp, ip, ip, vp, vp],
         "spi_simulate": [sa, vp, ip, ip, vp, vp, ip, vp, vp],
+        "spi_fim_rollout": [sa, vp, ip, ip, ip, vp, vp, ip, fp, ip, ip, vp, vp],
         "spi_cem_sample": [vp, vp, vp, vp, ip, ip, C.c_ulonglong, C.c_uint, vp, ip, vp],
         "spi_cem_update": [vp, vp, ip, ip, ip, ip, fp, vp, vp, vp, vp, vp, vp, vp],
     }

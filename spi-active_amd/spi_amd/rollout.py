@@ -96,6 +96,17 @@ class RolloutEngine:
                                      _lib.ptr(dof_state), _lib.ptr(torques), _lib.ptr(foot_force), _lib.stream())
         _lib.check(rc, "spi_physics_step")
 
+    def forward_dynamics(self, params: torch.Tensor, root: torch.Tensor, dof_state: torch.Tensor,
+                         torques: torch.Tensor, foot_force: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """[n, 18]: base spatial acceleration (base coords, angular first) and qdd."""
+        n = root.shape[0]
+        out = torch.empty((n, 18), dtype=torch.float32, device=self.device)
+        rc = self.L.spi_forward_dynamics(C_ref(self.args), _lib.ptr(params), params.shape[1], n, _lib.ptr(root),
+                                         _lib.ptr(dof_state), _lib.ptr(torques), _lib.ptr(out), _lib.ptr(foot_force),
+                                         _lib.stream())
+        _lib.check(rc, "spi_forward_dynamics")
+        return out
+
     def compute_torques(self, params: torch.Tensor, actions: torch.Tensor, dof_state: torch.Tensor,
                         out: Optional[torch.Tensor] = None) -> torch.Tensor:
         n = actions.shape[0]

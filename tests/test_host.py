@@ -124,3 +124,18 @@ def test_sinusoid_actions_amplitude():
     target = 0.25 * a                      # PD target offsets
     peak = np.abs(target).max(axis=1)
     assert peak.min() > 0.09 and peak.max() <= 0.4 + 1e-6
+
+
+def test_config_composition():
+    """Hydra-style defaults, _global_ packages and interpolation of the package configs."""
+    from spi_amd.config import compose, get_class
+    cfg = compose("active_sysid")
+    assert cfg.simulator._target_ == "spi_amd.simulator.hipsim_active_sysid.HipSimActiveSysId"
+    assert cfg.simulator.config.sim.fps == 200 and cfg.simulator.config.sim.control_decimation == 4
+    assert cfg.robot.dof_names[0] == "FL_hip_joint" and len(cfg.robot.body_names) == 19
+    assert cfg.env.config.exploration_params == ["mass"] and cfg.env.config.delta_param == 0.1
+    from spi_amd.params import SimParams
+    sp = SimParams.from_robot_config(cfg.robot, {"sim": dict(cfg.simulator.config.sim)})
+    assert sp.dt == 0.005 and sp.decimation == 4 and sp.substeps == 4
+    assert np.allclose(sp.kp, 20.0) and np.allclose(sp.kd, 0.5)
+    assert get_class("spi_amd.config.AttrDict").__name__ == "AttrDict"

@@ -1,0 +1,78 @@
+"""The kernel's specialised algorithm (spi_dyn.h) built for the host in FP32, against the FP64
+generic-tree oracle (CPU only).  Catches algorithm errors without a GPU; the GPU parity tests
+then check the device build of the same code."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+SRC = os.path.join(ROOT, "tests", "host_dyn", "spi_dyn_host.cpp")
+
+
+@pytest.fixture(scope="module")
+def host_lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("hostdyn") / "libspi_dyn_host.so")
+    cmd = ["/opt/rocm/bin/hipcc", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "spi-active_amd", "csrc"), SRC, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("hipcc host build unavailable: " + r.stderr[-300:])
+    L = ctypes.CDLL(out)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.spi_host_forward_dynamics.argtypes = [ctypes.c_void_p, fp, fp, fp, fp, fp]
+    L.spi_host_control_step.argtypes = [ctypes.c_void_p, fp, fp, fp]
+    return L
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def test_host_forward_dynamics_matches_oracle(host_lib, model, sim, theta0):
+    import spio
+    from spi_amd._lib import sim_args
+    from test_gpu_parity import _rand_params, _realistic_states
+    o = spio.Oracle(model, sim)
+    args = sim_args(sim)
+    rng = np.random.default_rng(21)
+    n = 64
+    states = _f(_realistic_states(model, sim, theta0, n, seed=22))
+    rows = _f(_rand_params(rng, theta0, n))
+    tau = _f(rng.normal(0, 5, (n, 12)))
+    worst = 0.0
+    for i in range(n):
+        acc, ff = np.zeros(18, np.float32), np.zeros(12, np.float32)
+        host_lib.spi_host_forward_dynamics(ctypes.byref(args), _p(rows[i]), _p(states[i]), _p(tau[i]), _p(acc), _p(ff))
+        a0, qdd, fo, _ = o.forward(states[i, :13], states[i, 13:25], states[i, 25:37], tau[i], rows[i])
+        ref = np.r_[a0, qdd]
+        worst = max(worst, np.abs(acc - ref).max() / max(np.abs(ref).max(), 1.0))
+        assert np.allclose(ff.reshape(4, 3), fo, rtol=1e-4, atol=1e-3)
+    assert worst <= 1e-3, worst
+
+
+def test_host_rollout_matches_oracle(host_lib, model, sim, theta0):
+    import spio
+    from spi_amd._lib import sim_args
+    from spi_amd.data import sinusoid_actions
+    from test_gpu_parity import _realistic_states
+    o = spio.Oracle(model, sim)
+    args = sim_args(sim)
+    rng = np.random.default_rng(23)
+    s0 = _f(_realistic_states(model, sim, theta0, 4, seed=24)[2])
+    s0[25:37] *= 0.2
+    acts = sinusoid_actions(rng, 1, 10, 0.02, 0.25)[0]
+    ref, rc = o.simulate(theta0, s0, acts)
+    st = s0.copy()
+    par = _f(theta0)
+    for t in range(10):
+        host_lib.spi_host_control_step(ctypes.byref(args), _p(par), _p(st), _p(_f(acts[t])))
+        assert np.abs(st[:7] - ref[t, :7]).max() < 1e-4
+        assert np.abs(st[13:25] - ref[t, 13:25]).max() < 1e-3
