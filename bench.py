@@ -30,9 +30,9 @@ METRIC = "env-steps/s, sysID solves/s at ≤1% param error; p50/p99 solve latenc
 # spi-active_amd/csrc/spi_flop_census.cpp (counting scalar over the kernel's own code).
 FLOPS_PER_CONTROL_STEP = 83828.9
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 (vector) peak
-# HBM bytes per rollout_cost_kernel launch (FETCH_SIZE + WRITE_SIZE, rocprofv3 --pmc, scaled
-# by a copy-kernel calibration); None until measured for this workload.
-TRAFFIC_BYTES_PER_LAUNCH = None
+# HBM bytes per rollout_cost_kernel launch of this workload: rocprofv3 --pmc FETCH_SIZE
+# (2,354.9 KiB, x 2.000 from a 256 MiB copy's reading) + WRITE_SIZE (4,096.0 KiB, x 1.000).
+TRAFFIC_BYTES_PER_LAUNCH = 9016514
 
 N_SAMPLES, N_SEG, HORIZON, ITERS = 4096, 256, 20, 12
 
@@ -43,7 +43,8 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-candidates", type=int, default=64)
+    p.add_argument("--cpu-candidates", type=int, default=256)
+    p.add_argument("--no-plain-baseline", action="store_true")
     p.add_argument("--profile-launch", action="store_true",
                    help="run one rollout_cost launch of the workload and a 256 MiB device copy (for rocprofv3 "
                         "--pmc FETCH_SIZE / WRITE_SIZE calibration), then exit")
@@ -141,6 +142,11 @@ def main():
                 "kernel": "rollout_cost_kernel", "kernel_ms_avg": round(float(kern.mean()), 3),
                 "flops_per_control_step": FLOPS_PER_CONTROL_STEP, "control_steps_per_launch": launch_ctrl}
 
+    plain = None
+    if not args.no_plain_baseline:
+        plain = plain_gpu_baseline(model, sim, segs, prior, eng.weights)
+        plain["fused_speedup"] = round(value / plain["value"], 1)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, sim, segs, prior, args.cpu_candidates)
@@ -159,9 +165,35 @@ def main():
         "param_error": {"base_mass_rel": round(mass_err, 5),
                         "max_rel_nonzero_params": round(float(rel[well].max()), 5), "per_param": param_err},
         "roofline": roofline,
+        "plain_gpu_baseline": plain,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def plain_gpu_baseline(model, sim, segs, prior, weights, steps=2):
+    """The same rollouts as PyTorch eager ops (Python loop over control steps, decimation and
+    sub-intervals; spi_amd.baselines.plain_torch) on all 4096 x 256 lanes of one CEM iteration,
+    timed over ``steps`` control steps."""
+    import torch
+    from spi_amd.baselines.plain_torch import PlainTorchRollout
+    pl = PlainTorchRollout(sim, model, weights=weights, device="cuda")
+    rng = np.random.default_rng(0)
+    rows = np.repeat(prior.mean[None], N_SAMPLES, axis=0)
+    rows[:, prior.free] += rng.normal(size=(N_SAMPLES, int(prior.free.sum()))) * prior.std[prior.free]
+    theta = torch.tensor(np.ascontiguousarray(np.clip(rows, prior.lo, prior.hi).T), dtype=torch.float32,
+                         device="cuda")
+    f = lambda x: torch.tensor(x, dtype=torch.float32, device="cuda")
+    init, acts, tgts = f(segs.init), f(segs.actions), f(segs.targets)
+    pl.segment_costs(theta[:, :64], init[:8], acts[:8], tgts[:8], steps=1)          # warm
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    pl.segment_costs(theta, init, acts, tgts, steps=steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t
+    n = N_SAMPLES * segs.n_seg * steps
+    return {"value": round(n / dt, 1), "unit": "env-steps/s",
+            "sample": f"{N_SAMPLES} candidates x {segs.n_seg} segments x {steps} control steps ({dt:.2f} s)"}
 
 
 def cpu_baseline(model, sim, segs, prior, n_cand):

@@ -78,6 +78,33 @@ typedef struct SpiCostArgs {
   float w[6];          /* weights: pos, quat, linvel, angvel, q, qd (reference mode uses 0, 1, 4) */
 } SpiCostArgs;
 
+/* One problem of a batched CEM launch (device-resident array of these).  Every pointer is
+ * device memory; `iteration` is advanced on the device by spi_batch_cem_update, so a captured
+ * graph of one iteration can be replayed.  theta/mean/std/lo/hi/min_std/best cover all
+ * SPI_P_COUNT parameters (fixed ones have std 0 and min_std 0). */
+typedef struct SpiBatchProblem {
+  const float* segments;        /* [n_seg][spi_segment_stride(horizon)] */
+  const float* seg_weight;      /* [n_seg]: cost[c] = sum_s seg_weight[s] * cost_seg[s][c] */
+  int n_seg;
+  int horizon;
+  SpiCostArgs cost;
+  int n_samples;                /* <= SPI_MAX_CEM_SAMPLES */
+  int n_elite;
+  float alpha;
+  unsigned int iteration;
+  unsigned long long seed;
+  float* mean;
+  float* std;
+  const float* lo;
+  const float* hi;
+  const float* min_std;
+  float* theta;                 /* [SPI_P_COUNT][n_samples] */
+  float* cost_seg;              /* [n_seg][n_samples] */
+  float* cost_out;              /* [n_samples] */
+  float* best;                  /* [SPI_P_COUNT] */
+  float* best_cost;             /* [1] */
+} SpiBatchProblem;
+
 const char* spi_last_error(void);
 int spi_abi_version(void);
 
@@ -176,6 +203,25 @@ int spi_cem_sample(const float* mean_dev, const float* std_dev, const float* lo_
 int spi_cem_update(const float* cost_dev, const float* theta_dev, int ld, int n_param, int n, int n_elite,
                    float alpha, const float* min_std_dev, float* mean_dev, float* std_dev, float* best_dev,
                    float* best_cost_dev, int* elite_idx_dev, hipStream_t stream);
+
+/* ---- batched problems (many independent sysID problems per launch) ---- */
+
+/* Sample every problem's candidates for its current iteration (as spi_cem_sample). */
+int spi_batch_cem_sample(const SpiBatchProblem* probs_dev, int n_prob, int max_samples, hipStream_t stream);
+
+/* Fused rollout + cost for every (problem, segment, candidate block) task with a persistent
+ * grid of n_blocks workgroups of `block` lanes (64, 128 or 256; unroll 1 or 2 of the physics
+ * step loop) pulling tasks from queue_dev (2 x u64, zero before first use; the launch leaves
+ * it zero).  task_prefix_dev[p] = number of tasks of problems 0..p-1, a task of problem p
+ * being (segment, block of `block` candidates); list long horizons first for balance.
+ * Replaces the per-problem Python loops of scripts/mass_opt.py:203-219. */
+int spi_batch_rollout_cost(const SpiSimArgs* sim, const SpiBatchProblem* probs_dev, const long long* task_prefix_dev,
+                           int n_prob, long long n_tasks, int max_horizon, int block, int unroll, int n_blocks,
+                           unsigned long long* queue_dev, hipStream_t stream);
+
+/* Per problem (one workgroup each): cost_out = weighted sum over segments (fixed order), then
+ * the CEM update of spi_cem_update, then iteration += 1. */
+int spi_batch_cem_update(const SpiBatchProblem* probs_dev, int n_prob, hipStream_t stream);
 
 #ifdef __cplusplus
 }

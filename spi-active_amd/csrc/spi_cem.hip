@@ -88,17 +88,16 @@ __global__ void reduce_segments_kernel(const float* __restrict__ cs, const float
 
 constexpr int kCemThreads = 1024;
 
-// Single workgroup: (cost, index) pairs sorted ascending in LDS (bitonic network, ties broken
-// by index, NaN treated as +inf), then per-parameter elite moments with wave reductions.
-__global__ __launch_bounds__(kCemThreads) void cem_update_kernel(
-    const float* __restrict__ cost, const float* __restrict__ theta, int ld, int n_param, int n, int n_elite,
-    float alpha, const float* __restrict__ min_std, float* __restrict__ mean, float* __restrict__ stdv,
-    float* __restrict__ best, float* __restrict__ best_cost, int* __restrict__ elite_idx, int npow2) {
-  __shared__ float key[SPI_MAX_CEM_SAMPLES];
-  __shared__ int val[SPI_MAX_CEM_SAMPLES];
+// (cost, index) pairs sorted ascending in LDS (bitonic network, ties broken by index, NaN
+// treated as +inf), then per-parameter elite moments with wave reductions and the best-so-far.
+// Called by one whole workgroup of kCemThreads threads; keys must already hold the costs.
+__device__ void cem_sort_update(float* key, int* val, int n, int npow2, const float* __restrict__ theta, int ld,
+                                int n_param, int n_elite, float alpha, const float* __restrict__ min_std,
+                                float* __restrict__ mean, float* __restrict__ stdv, float* __restrict__ best,
+                                float* __restrict__ best_cost, int* __restrict__ elite_idx) {
   const int tid = threadIdx.x;
   for (int i = tid; i < npow2; i += kCemThreads) {
-    float c = i < n ? cost[i] : INFINITY;
+    float c = i < n ? key[i] : INFINITY;
     key[i] = (c == c) ? c : INFINITY;
     val[i] = i;
   }
@@ -152,6 +151,63 @@ __global__ __launch_bounds__(kCemThreads) void cem_update_kernel(
     for (int p = tid; p < n_param; p += kCemThreads) best[p] = theta[static_cast<size_t>(p) * ld + val[0]];
     if (tid == 0) best_cost[0] = key[0];
   }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kCemThreads) void cem_update_kernel(
+    const float* __restrict__ cost, const float* __restrict__ theta, int ld, int n_param, int n, int n_elite,
+    float alpha, const float* __restrict__ min_std, float* __restrict__ mean, float* __restrict__ stdv,
+    float* __restrict__ best, float* __restrict__ best_cost, int* __restrict__ elite_idx, int npow2) {
+  __shared__ float key[SPI_MAX_CEM_SAMPLES];
+  __shared__ int val[SPI_MAX_CEM_SAMPLES];
+  for (int i = threadIdx.x; i < n; i += kCemThreads) key[i] = cost[i];
+  __syncthreads();
+  cem_sort_update(key, val, n, npow2, theta, ld, n_param, n_elite, alpha, min_std, mean, stdv, best, best_cost,
+                  elite_idx);
+}
+
+// Batched: blockIdx.y = problem; four normals per thread as in cem_sample_kernel.
+__global__ void cem_sample_batch_kernel(const SpiBatchProblem* __restrict__ probs) {
+  const SpiBatchProblem& P = probs[blockIdx.y];
+  const unsigned long long blk = static_cast<unsigned long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const unsigned long long total = static_cast<unsigned long long>(SPI_P_COUNT) * P.n_samples;
+  const unsigned long long i0 = blk * 4;
+  if (i0 >= total) return;
+  float z[4];
+  philox_normals(P.seed, P.iteration, blk, z);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    unsigned long long idx = i0 + e;
+    if (idx >= total) break;
+    int p = static_cast<int>(idx / P.n_samples), k = static_cast<int>(idx % P.n_samples);
+    float v = P.mean[p] + P.std[p] * z[e];
+    v = fminf(fmaxf(v, P.lo[p]), P.hi[p]);
+    P.theta[static_cast<size_t>(p) * P.n_samples + k] = v;
+  }
+}
+
+// Batched: one workgroup per problem; segment reduction (fixed order), sort, moments, best,
+// iteration + 1.
+__global__ __launch_bounds__(kCemThreads) void cem_update_batch_kernel(SpiBatchProblem* __restrict__ probs) {
+  __shared__ float key[SPI_MAX_CEM_SAMPLES];
+  __shared__ int val[SPI_MAX_CEM_SAMPLES];
+  SpiBatchProblem& P = probs[blockIdx.x];
+  const int n = P.n_samples;
+  for (int c = threadIdx.x; c < n; c += kCemThreads) {
+    float acc = 0.0f;
+    for (int s = 0; s < P.n_seg; ++s) {
+      float ws = P.seg_weight[s];
+      if (ws != 0.0f) acc += ws * P.cost_seg[static_cast<size_t>(s) * n + c];
+    }
+    P.cost_out[c] = acc;
+    key[c] = acc;
+  }
+  __syncthreads();
+  int npow2 = 2;
+  while (npow2 < n) npow2 <<= 1;
+  cem_sort_update(key, val, n, npow2, P.theta, n, SPI_P_COUNT, P.n_elite, P.alpha, P.min_std, P.mean, P.std,
+                  P.best, P.best_cost, nullptr);
+  if (threadIdx.x == 0) P.iteration = P.iteration + 1u;
 }
 
 }  // namespace
@@ -199,6 +255,26 @@ int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, 
   if (n_cand == 0) return SPI_OK;
   reduce_segments_kernel<<<(n_cand + 255) / 256, 256, 0, stream>>>(cost_seg_dev, seg_weight_dev, n_seg, n_cand,
                                                                    cost_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
+  return SPI_OK;
+}
+
+int spi_batch_cem_sample(const SpiBatchProblem* probs_dev, int n_prob, int max_samples, hipStream_t stream) {
+  if (n_prob < 1 || max_samples < 1 || max_samples > SPI_MAX_CEM_SAMPLES || !probs_dev)
+    return cem_fail(SPI_ERR_INVALID_ARG, "spi_batch_cem_sample: bad arguments");
+  if (n_prob > 65535) return cem_fail(SPI_ERR_TOO_LARGE, "spi_batch_cem_sample: too many problems");
+  unsigned long long blocks = (static_cast<unsigned long long>(SPI_P_COUNT) * max_samples + 3) / 4;
+  dim3 grid(static_cast<unsigned>((blocks + 255) / 256), static_cast<unsigned>(n_prob));
+  cem_sample_batch_kernel<<<grid, 256, 0, stream>>>(probs_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
+  return SPI_OK;
+}
+
+int spi_batch_cem_update(const SpiBatchProblem* probs_dev, int n_prob, hipStream_t stream) {
+  if (n_prob < 1 || !probs_dev) return cem_fail(SPI_ERR_INVALID_ARG, "spi_batch_cem_update: bad arguments");
+  cem_update_batch_kernel<<<n_prob, kCemThreads, 0, stream>>>(const_cast<SpiBatchProblem*>(probs_dev));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
   return SPI_OK;

@@ -60,6 +60,35 @@ __device__ __forceinline__ bool finite_bits(float x) { return (__float_as_uint(x
 
 constexpr int kRolloutThreads = 256;
 
+// One (candidate, segment) rollout of H control steps from the segment record `rec` (init
+// state, then per step: action, target), returning the cost as float bits (+inf bits when
+// the lane diverged).  `rec` is in LDS and read at uniform addresses.
+template <int UNROLL>
+__device__ __forceinline__ unsigned int rollout_lane(const SpiSimArgs& a, const SpiCostArgs& cost,
+                                                     const LaneParams& lp, const float* rec0, int H) {
+  State st;
+  state_from_root(st, rec0, rec0 + 13, rec0 + 25, 1);
+  float acc = 0.0f;
+  bool ok = true;
+  for (int t = 0; t < H; ++t) {
+    const float* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+    float act[12];
+#pragma unroll
+    for (int j = 0; j < 12; ++j) act[j] = rec[j];
+#pragma unroll UNROLL
+    for (int d = 0; d < a.decimation; ++d) {
+      float tau[12];
+      control_torques(a, lp, act, st, tau);
+      physics_step(a, lp, st, tau, nullptr);
+    }
+    if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost(cost, st, rec + 12);
+    const float nrm = state_norm(st);
+    if (!finite_bits(nrm) || nrm >= a.state_limit) { ok = false; break; }
+  }
+  // diverged lanes get +inf, as a bit pattern (this file is built with finite-math)
+  return (ok && finite_bits(acc)) ? __float_as_uint(acc) : 0x7f800000u;
+}
+
 // One workgroup = one segment x 256 candidates.  The segment record is staged once in LDS
 // and read by every lane (uniform address: LDS broadcast); parameters are read coalesced
 // from the SoA array.
@@ -75,27 +104,54 @@ __global__ __launch_bounds__(kRolloutThreads) void rollout_cost_kernel(
   const int c = cb * kRolloutThreads + threadIdx.x;
   if (c >= n_cand) return;
   const LaneParams lp = load_params(par, ld, c);
-  State st;
-  state_from_root(st, lds, lds + 13, lds + 25, 1);
-  float acc = 0.0f;
-  bool ok = true;
-  for (int t = 0; t < H; ++t) {
-    const float* rec = lds + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
-    float act[12];
-#pragma unroll
-    for (int j = 0; j < 12; ++j) act[j] = rec[j];
-    for (int d = 0; d < a.decimation; ++d) {
-      float tau[12];
-      control_torques(a, lp, act, st, tau);
-      physics_step(a, lp, st, tau, nullptr);
+  reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] = rollout_lane<1>(a, cost, lp, lds, H);
+}
+
+// Batched problems: a persistent grid pulls tasks (problem, segment, candidate block) from a
+// device queue.  Tickets are handed out in table order (the host orders problems longest
+// horizon first), every block leaves its loop once the tickets run out, and the last block
+// to leave resets the queue so the same launch can be replayed (HIP graphs).
+template <int BLOCK, int UNROLL>
+__global__ __launch_bounds__(BLOCK) void rollout_cost_batch_kernel(SpiSimArgs a, const SpiBatchProblem* __restrict__ probs,
+                                                                   const long long* __restrict__ prefix, int n_prob,
+                                                                   long long n_tasks,
+                                                                   unsigned long long* __restrict__ queue) {
+  extern __shared__ float lds[];
+  __shared__ long long ticket;
+  for (;;) {
+    if (threadIdx.x == 0) ticket = static_cast<long long>(atomicAdd(&queue[0], 1ULL));
+    __syncthreads();
+    const long long t = ticket;
+    if (t >= n_tasks) break;
+    int lo = 0, hi = n_prob - 1;             // last problem with prefix[p] <= t
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= t) lo = mid; else hi = mid - 1;
     }
-    if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost(cost, st, rec + 12);
-    const float nrm = state_norm(st);
-    if (!finite_bits(nrm) || nrm >= a.state_limit) { ok = false; break; }
+    const SpiBatchProblem& P = probs[lo];
+    const long long local = t - prefix[lo];
+    const int n_cblk = (P.n_samples + BLOCK - 1) / BLOCK;
+    const int s = static_cast<int>(local / n_cblk), cb = static_cast<int>(local % n_cblk);
+    const int stride = SPI_STATE_DIM + P.horizon * (12 + SPI_STATE_DIM);
+    const float* src = P.segments + static_cast<size_t>(s) * stride;
+    for (int i = threadIdx.x; i < stride; i += BLOCK) lds[i] = src[i];
+    __syncthreads();
+    const int c = cb * BLOCK + threadIdx.x;
+    if (c < P.n_samples) {
+      const LaneParams lp = load_params(P.theta, P.n_samples, c);
+      reinterpret_cast<unsigned int*>(P.cost_seg)[static_cast<size_t>(s) * P.n_samples + c] =
+          rollout_lane<UNROLL>(a, P.cost, lp, lds, P.horizon);
+    }
+    __syncthreads();                          // LDS and `ticket` are reused by the next task
   }
-  // diverged lanes get +inf, written as a bit pattern (this file is built with finite-math)
-  reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] =
-      (ok && finite_bits(acc)) ? __float_as_uint(acc) : 0x7f800000u;
+  if (threadIdx.x == 0) {
+    __threadfence();                          // this block's last ticket precedes its exit count
+    unsigned long long done = atomicAdd(&queue[1], 1ULL);
+    if (done == gridDim.x - 1) {              // every other block has left its loop
+      atomicExch(&queue[0], 0ULL);
+      atomicExch(&queue[1], 0ULL);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void physics_step_kernel(SpiSimArgs a, const float* __restrict__ par, int ld,
@@ -499,6 +555,29 @@ int spi_fim_rollout(const SpiSimArgs* sim, const float* params_dev, int ld, int 
   fim_rollout_kernel<<<blocks, 256, 0, stream>>>(*sim, params_dev, ld, n_group, n_param, G2, init_dev, actions_dev,
                                                  n_steps, delta, ksync, ksync_mode, fim_dev);
   return launch_check("fim_rollout_kernel");
+}
+
+int spi_batch_rollout_cost(const SpiSimArgs* sim, const SpiBatchProblem* probs_dev, const long long* task_prefix_dev,
+                           int n_prob, long long n_tasks, int max_horizon, int block, int unroll, int n_blocks,
+                           unsigned long long* queue_dev, hipStream_t stream) {
+  if (!sim_args_ok(sim)) return fail(SPI_ERR_INVALID_ARG, "spi_batch_rollout_cost: invalid SpiSimArgs");
+  if (n_prob < 1 || n_tasks < 0 || max_horizon < 1 || n_blocks < 1 || !probs_dev || !task_prefix_dev || !queue_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_batch_rollout_cost: bad arguments");
+  if (n_tasks == 0) return SPI_OK;
+  const size_t lds = static_cast<size_t>(spi_segment_stride(max_horizon)) * sizeof(float);
+  if (lds > 64 * 1024) return fail(SPI_ERR_TOO_LARGE, "spi_batch_rollout_cost: horizon too long for LDS staging");
+#define SPI_BATCH_LAUNCH(B, U)                                                                               \
+  rollout_cost_batch_kernel<B, U><<<n_blocks, B, lds, stream>>>(*sim, probs_dev, task_prefix_dev, n_prob, n_tasks, \
+                                                                queue_dev)
+  if (block == 64 && unroll == 1) SPI_BATCH_LAUNCH(64, 1);
+  else if (block == 128 && unroll == 1) SPI_BATCH_LAUNCH(128, 1);
+  else if (block == 256 && unroll == 1) SPI_BATCH_LAUNCH(256, 1);
+  else if (block == 64 && unroll == 2) SPI_BATCH_LAUNCH(64, 2);
+  else if (block == 128 && unroll == 2) SPI_BATCH_LAUNCH(128, 2);
+  else if (block == 256 && unroll == 2) SPI_BATCH_LAUNCH(256, 2);
+  else return fail(SPI_ERR_INVALID_ARG, "spi_batch_rollout_cost: block must be 64/128/256, unroll 1/2");
+#undef SPI_BATCH_LAUNCH
+  return launch_check("rollout_cost_batch_kernel");
 }
 
 int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, const float* init_dev,

@@ -92,6 +92,9 @@ def lib() -> C.CDLL:
         "spi_fim_rollout": [sa, vp, ip, ip, ip, vp, vp, ip, fp, ip, ip, vp, vp],
         "spi_cem_sample": [vp, vp, vp, vp, ip, ip, C.c_ulonglong, C.c_uint, vp, ip, vp],
         "spi_cem_update": [vp, vp, ip, ip, ip, ip, fp, vp, vp, vp, vp, vp, vp, vp],
+        "spi_batch_cem_sample": [vp, ip, ip, vp],
+        "spi_batch_rollout_cost": [sa, vp, vp, ip, C.c_longlong, ip, ip, ip, ip, vp, vp],
+        "spi_batch_cem_update": [vp, ip, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

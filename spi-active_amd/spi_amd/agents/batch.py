@@ -1,0 +1,192 @@
+"""Many independent CEM sysID problems in one batched launch per CEM phase.
+
+Each problem has its own segments, horizon, sample count, prior and Philox stream.  One CEM
+iteration of the whole batch is three kernels -- ``spi_batch_cem_sample``,
+``spi_batch_rollout_cost`` (a persistent grid pulling (problem, segment, candidate-block)
+tasks from a device queue, longest horizons first) and ``spi_batch_cem_update`` (segment
+reduction, elite sort, moments, iteration counter, one workgroup per problem) -- and can be
+captured in a HIP graph and replayed, since every per-iteration quantity lives on the device.
+A problem's results do not depend on which other problems share the batch.
+
+The reference solves one problem at a time: a Python loop over candidate masses
+(scripts/mass_landscape.py:123-126) or Optuna trials (scripts/mass_opt.py:203-219).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..params import P_COUNT
+from ..rollout import C_ref, RolloutEngine
+from .sysid import Prior
+
+
+class SpiBatchProblem(C.Structure):
+    _fields_ = [("segments", C.c_void_p), ("seg_weight", C.c_void_p), ("n_seg", C.c_int), ("horizon", C.c_int),
+                ("cost", _lib.SpiCostArgs), ("n_samples", C.c_int), ("n_elite", C.c_int), ("alpha", C.c_float),
+                ("iteration", C.c_uint), ("seed", C.c_ulonglong), ("mean", C.c_void_p), ("std", C.c_void_p),
+                ("lo", C.c_void_p), ("hi", C.c_void_p), ("min_std", C.c_void_p), ("theta", C.c_void_p),
+                ("cost_seg", C.c_void_p), ("cost_out", C.c_void_p), ("best", C.c_void_p), ("best_cost", C.c_void_p)]
+
+
+@dataclass
+class ProblemSpec:
+    segments: torch.Tensor            # [n_seg, stride(H)] device
+    horizon: int
+    prior: Prior
+    n_samples: int = 4096
+    elite_frac: float = 0.1
+    alpha: float = 0.0
+    seed: int = 0
+    seg_weights: Optional[torch.Tensor] = None
+
+
+class _Buffers:
+    """Device state of one problem (CEM moments, samples, costs, best so far)."""
+    def __init__(self, spec: ProblemSpec, dev):
+        n, n_seg = spec.n_samples, spec.segments.shape[0]
+        f = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).contiguous()
+        self.mean, self.std = f(spec.prior.mean).clone(), f(spec.prior.std).clone()
+        self.lo, self.hi, self.min_std = f(spec.prior.lo), f(spec.prior.hi), f(spec.prior.min_std)
+        self.theta = torch.empty((P_COUNT, n), dtype=torch.float32, device=dev)
+        self.cost_seg = torch.empty((n_seg, n), dtype=torch.float32, device=dev)
+        self.cost = torch.empty(n, dtype=torch.float32, device=dev)
+        self.best = self.mean.clone()
+        self.best_cost = torch.full((1,), float("inf"), dtype=torch.float32, device=dev)
+        self.seg_w = (spec.seg_weights if spec.seg_weights is not None else
+                      torch.full((n_seg,), 1.0 / n_seg, dtype=torch.float32, device=dev))
+
+    @staticmethod
+    def footprint(spec: ProblemSpec) -> int:
+        """Device bytes a problem holds while admitted (its segments included)."""
+        n, n_seg = spec.n_samples, spec.segments.shape[0]
+        return 4 * (P_COUNT * n + n_seg * n + n + 7 * P_COUNT + n_seg) + spec.segments.numel() * 4
+
+
+class BatchCEM:
+    """A fixed set of problems solved together; ``iterate()`` runs one CEM iteration of all."""
+
+    def __init__(self, engine: RolloutEngine, specs: Sequence[ProblemSpec], block: int = 256, unroll: int = 1,
+                 blocks_per_cu: int = 4, n_cu: int = 256, buffers: Optional[Sequence["_Buffers"]] = None,
+                 iterations_done: Optional[Sequence[int]] = None):
+        if not specs:
+            raise ValueError("no problems")
+        self.engine = engine
+        self.dev = engine.device
+        # longest horizon first: the persistent grid takes tickets in table order
+        self.order = sorted(range(len(specs)), key=lambda i: -specs[i].horizon)
+        self.specs = [specs[i] for i in self.order]
+        if buffers is None:
+            buffers = [_Buffers(s, self.dev) for s in specs]
+        self.bufs = [buffers[i] for i in self.order]
+        done = list(iterations_done) if iterations_done is not None else [0] * len(specs)
+        self.start_iter = [done[i] for i in self.order]
+        self.block, self.unroll = block, unroll
+        self.n_blocks = blocks_per_cu * n_cu
+        self.queue = torch.zeros(2, dtype=torch.int64, device=self.dev)
+        self.max_h = max(s.horizon for s in self.specs)
+        self.max_n = max(s.n_samples for s in self.specs)
+        self._build_table()
+
+    def _build_table(self):
+        arr = (SpiBatchProblem * len(self.specs))()
+        prefix, tot = [], 0
+        for i, (s, b) in enumerate(zip(self.specs, self.bufs)):
+            p = arr[i]
+            p.segments, p.seg_weight = _lib.ptr(s.segments), _lib.ptr(b.seg_w)
+            p.n_seg, p.horizon = s.segments.shape[0], s.horizon
+            p.cost = self.engine.cargs
+            p.n_samples = s.n_samples
+            p.n_elite = max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9)))
+            p.alpha, p.iteration, p.seed = s.alpha, self.start_iter[i], s.seed
+            p.mean, p.std, p.lo, p.hi = _lib.ptr(b.mean), _lib.ptr(b.std), _lib.ptr(b.lo), _lib.ptr(b.hi)
+            p.min_std, p.theta, p.cost_seg = _lib.ptr(b.min_std), _lib.ptr(b.theta), _lib.ptr(b.cost_seg)
+            p.cost_out, p.best, p.best_cost = _lib.ptr(b.cost), _lib.ptr(b.best), _lib.ptr(b.best_cost)
+            prefix.append(tot)
+            tot += p.n_seg * ((s.n_samples + self.block - 1) // self.block)
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+        self.table = torch.from_numpy(raw.copy()).to(self.dev)
+        self.prefix = torch.tensor(prefix, dtype=torch.int64, device=self.dev)
+        self.n_tasks = tot
+
+    def set_launch(self, block: int, unroll: int) -> None:
+        if block != self.block:
+            self.block = block
+            self._build_table()
+        self.unroll = unroll
+
+    # ---------------------------------------------------------------- phases
+    def sample(self):
+        _lib.check(_lib.lib().spi_batch_cem_sample(self.table.data_ptr(), len(self.specs), self.max_n, _lib.stream()),
+                   "spi_batch_cem_sample")
+
+    def rollout(self):
+        rc = _lib.lib().spi_batch_rollout_cost(C_ref(self.engine.args), self.table.data_ptr(), self.prefix.data_ptr(),
+                                               len(self.specs), self.n_tasks, self.max_h, self.block, self.unroll,
+                                               self.n_blocks, self.queue.data_ptr(), _lib.stream())
+        _lib.check(rc, "spi_batch_rollout_cost")
+
+    def update(self):
+        _lib.check(_lib.lib().spi_batch_cem_update(self.table.data_ptr(), len(self.specs), _lib.stream()),
+                   "spi_batch_cem_update")
+
+    def iterate(self):
+        self.sample()
+        self.rollout()
+        self.update()
+
+    def solve(self, iterations: int, graph: bool = True):
+        if not graph:
+            for _ in range(iterations):
+                self.iterate()
+            return
+        g = self.capture()
+        for _ in range(iterations):
+            g.replay()
+
+    def capture(self) -> "torch.cuda.CUDAGraph":
+        """HIP graph of one iteration (the problem table's iteration counters advance on device)."""
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            # capture without running: the graph's first replay is the next iteration
+            with torch.cuda.graph(g, stream=s):
+                self.iterate()
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+    # ---------------------------------------------------------------- results (in the caller's order)
+    def results(self) -> List[Dict[str, np.ndarray]]:
+        out = [None] * len(self.specs)
+        for k, i in enumerate(self.order):
+            b = self.bufs[k]
+            out[i] = {"mean": b.mean.cpu().numpy(), "std": b.std.cpu().numpy(), "best": b.best.cpu().numpy(),
+                      "best_cost": float(b.best_cost.item()), "cost": b.cost.cpu().numpy()}
+        return out
+
+    # ---------------------------------------------------------------- autotuner
+    def autotune(self, candidates: Sequence[Tuple[int, int]] = ((64, 1), (128, 1), (256, 1), (64, 2), (128, 2),
+                                                                (256, 2)), repeats: int = 2) -> Dict:
+        """Time one rollout phase per (envs per block, physics-step unroll) and keep the fastest."""
+        self.sample()
+        times = {}
+        for blk, un in candidates:
+            self.set_launch(blk, un)
+            self.rollout()                      # warm
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                self.rollout()
+            torch.cuda.synchronize()
+            times[(blk, un)] = (time.perf_counter() - t0) / repeats
+        best = min(times, key=times.get)
+        self.set_launch(*best)
+        return {"best": best, "seconds": {f"{b}x{u}": round(t, 5) for (b, u), t in times.items()}}

@@ -30,6 +30,7 @@ class RolloutEngine:
         self.args = _lib.sim_args(sim, state_limit)
         if weights is None:
             weights = DEFAULT_TRACKING_WEIGHTS if cost_mode == "tracking" else REFERENCE_WEIGHTS
+        self.weights = [float(w) for w in weights]
         self.cost_mode = COST_MODES[cost_mode]
         self.cargs = _lib.cost_args(self.cost_mode, weights)
         self.L = _lib.lib()

@@ -1,0 +1,175 @@
+"""Continuous-batching service for sysID and excitation-scoring jobs on one GPU.
+
+* sysID jobs (a CEM problem: segments, horizon, sample count, iterations) are admitted as
+  soon as they have arrived and their device footprint fits the HBM budget; every tick runs
+  one CEM iteration of all admitted jobs in one batched launch per phase (agents/batch.py),
+  so a job that arrives while others are running joins at the next iteration boundary and
+  leaves when its own iteration count is reached.  CEM state stays on the device for the
+  whole life of a job; only membership changes rebuild the (small) problem table.
+* excitation-scoring jobs (FD Fisher information of candidate excitations,
+  agents/active.py) run on a second stream and overlap with the CEM iteration.
+
+A job's result does not depend on the other jobs in flight: each problem has its own Philox
+stream, iteration counter and buffers, and the batched kernels never mix problems.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .agents.active import FDFIMObjective
+from .agents.batch import BatchCEM, ProblemSpec, _Buffers
+from .rollout import RolloutEngine
+
+
+@dataclass
+class SysIdRequest:
+    job_id: int
+    arrival: float                 # seconds after the service started
+    spec: ProblemSpec
+    iterations: int = 12
+
+
+@dataclass
+class ScoreRequest:
+    job_id: int
+    arrival: float
+    actions: torch.Tensor          # [M, T, 12] candidate excitations
+    init: torch.Tensor             # [37] or [M * (P + 1), 37]
+    theta: np.ndarray
+    explore: Sequence[str] = ("mass",)
+    delta: float = 0.1
+
+
+@dataclass
+class _Active:
+    req: SysIdRequest
+    buf: _Buffers
+    done: int = 0
+
+
+@dataclass
+class ServiceStats:
+    latency_s: Dict[int, float] = field(default_factory=dict)
+    results: Dict[int, Dict] = field(default_factory=dict)
+    ticks: int = 0
+    max_active: int = 0
+    wall_s: float = 0.0
+
+    def summary(self) -> Dict:
+        lat = np.array(list(self.latency_s.values())) if self.latency_s else np.zeros(1)
+        return {"completed": len(self.latency_s), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                "jobs_per_s": len(self.latency_s) / max(self.wall_s, 1e-9), "ticks": self.ticks,
+                "max_active": self.max_active, "wall_s": self.wall_s}
+
+
+class SysIdServer:
+    def __init__(self, engine: RolloutEngine, hbm_budget_bytes: float = 200e9, max_active: int = 256,
+                 block: int = 256, unroll: int = 1):
+        self.engine = engine
+        self.budget = float(hbm_budget_bytes)
+        self.max_active = max_active
+        self.block, self.unroll = block, unroll
+        self.score_stream = torch.cuda.Stream()   # default priority, like every stream here
+
+    def run(self, requests: Sequence, clock: str = "wall") -> ServiceStats:
+        """Serve requests until all are done.
+
+        clock="wall": ``arrival`` is in seconds since the call; clock="ticks": ``arrival`` counts
+        CEM iterations of the service (deterministic membership, for tests and replays).  Latency
+        runs from the moment a request has arrived (queueing for memory included) to the moment
+        its result is on the host."""
+        if clock not in ("wall", "ticks"):
+            raise ValueError(f"clock must be 'wall' or 'ticks', not {clock!r}")
+        realtime = clock == "wall"
+        pending = sorted(requests, key=lambda r: r.arrival)
+        for r in pending:
+            if isinstance(r, SysIdRequest) and _Buffers.footprint(r.spec) > self.budget:
+                raise ValueError(f"job {r.job_id} needs {_Buffers.footprint(r.spec)} B, over the whole budget")
+        stats = ServiceStats()
+        active: List[_Active] = []
+        scoring: List = []
+        arrived: Dict[int, float] = {}
+        batch: Optional[BatchCEM] = None
+        used = 0.0
+        t0 = time.perf_counter()
+        i = 0
+        while i < len(pending) or active or scoring:
+            now = time.perf_counter() - t0
+            t_adm = now if realtime else stats.ticks
+            for r in pending[i:]:
+                if r.arrival > t_adm:
+                    break
+                arrived.setdefault(r.job_id, r.arrival if realtime else now)
+            changed = False
+            # admit in arrival order while the head of the queue fits
+            while i < len(pending) and pending[i].arrival <= t_adm:
+                r = pending[i]
+                if isinstance(r, ScoreRequest):
+                    scoring.append(self._launch_score(r))
+                    i += 1
+                    continue
+                need = _Buffers.footprint(r.spec)
+                if used + need > self.budget or len(active) >= self.max_active:
+                    break
+                active.append(_Active(r, _Buffers(r.spec, self.engine.device)))
+                used += need
+                changed = True
+                i += 1
+            if active:
+                if changed or batch is None:
+                    batch = BatchCEM(self.engine, [a.req.spec for a in active], block=self.block,
+                                     unroll=self.unroll, buffers=[a.buf for a in active],
+                                     iterations_done=[a.done for a in active])
+                batch.iterate()
+                for a in active:
+                    a.done += 1
+                stats.ticks += 1
+                stats.max_active = max(stats.max_active, len(active))
+            torch.cuda.synchronize()
+            now = time.perf_counter() - t0
+            keep = []
+            for a in active:
+                if a.done >= a.req.iterations:
+                    b = a.buf
+                    stats.results[a.req.job_id] = {"mean": b.mean.cpu().numpy(), "std": b.std.cpu().numpy(),
+                                                   "best": b.best.cpu().numpy(),
+                                                   "best_cost": float(b.best_cost.item())}
+                    stats.latency_s[a.req.job_id] = time.perf_counter() - t0 - arrived[a.req.job_id]
+                    used -= _Buffers.footprint(a.req.spec)
+                else:
+                    keep.append(a)
+            if len(keep) != len(active):
+                batch = None
+            active = keep
+            for job_id, out in scoring:          # the synchronize above covers the scoring stream
+                stats.results[job_id] = {"fim": out.cpu().numpy()}
+                stats.latency_s[job_id] = time.perf_counter() - t0 - arrived[job_id]
+            scoring = []
+            if not active and i < len(pending):
+                if realtime:
+                    wait = pending[i].arrival - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.01))
+                else:
+                    stats.ticks = max(stats.ticks, int(math.ceil(pending[i].arrival)))
+        stats.wall_s = time.perf_counter() - t0
+        return stats
+
+    def _launch_score(self, r: ScoreRequest):
+        self.score_stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.score_stream):
+            obj = FDFIMObjective(self.engine, r.theta, list(r.explore), delta=r.delta)
+            out = obj(r.actions, r.init)
+        return (r.job_id, out)
+
+
+def poisson_arrivals(n: int, rate_hz: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate_hz, size=n))

@@ -35,14 +35,16 @@ def test_library_exports_every_header_symbol():
 def test_ctypes_structs_match_c_layout():
     """sizeof/offsetof of SpiSimArgs and SpiCostArgs as compiled by the host compiler."""
     from spi_amd._lib import SpiCostArgs, SpiSimArgs
+    from spi_amd.agents.batch import SpiBatchProblem
     src = r"""
     #include <cstdio>
     #include <cstddef>
     #include "spi_hip.h"
     int main() {
-      std::printf("%zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs), offsetof(SpiSimArgs, effort),
+      std::printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs), offsetof(SpiSimArgs, effort),
                   offsetof(SpiSimArgs, motor_model), offsetof(SpiSimArgs, state_limit), sizeof(SpiCostArgs),
-                  offsetof(SpiCostArgs, w));
+                  offsetof(SpiCostArgs, w), sizeof(SpiBatchProblem), offsetof(SpiBatchProblem, seed),
+                  offsetof(SpiBatchProblem, mean), offsetof(SpiBatchProblem, best_cost));
     }
     """
     with tempfile.TemporaryDirectory() as d:
@@ -53,7 +55,9 @@ def test_ctypes_structs_match_c_layout():
                         "/opt/rocm/include", c, "-o", exe], check=True)
         got = [int(x) for x in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     expect = [ctypes.sizeof(SpiSimArgs), SpiSimArgs.effort.offset, SpiSimArgs.motor_model.offset,
-              SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset]
+              SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset,
+              ctypes.sizeof(SpiBatchProblem), SpiBatchProblem.seed.offset, SpiBatchProblem.mean.offset,
+              SpiBatchProblem.best_cost.offset]
     assert got == expect
 
 

@@ -1,0 +1,40 @@
+"""The plain PyTorch-eager rollout path (the GPU comparison baseline) reproduces the FP64
+oracle when run in float64 on the CPU, for every motor model and both cost modes."""
+import numpy as np
+import pytest
+import torch
+
+
+@pytest.fixture(scope="module")
+def segs_small(model, sim):
+    import spio
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    o = spio.Oracle(model, sim)
+    prior = identification_prior()
+
+    def simulate(theta, init, acts):
+        return np.stack([o.simulate(np.asarray(theta)[i], init[i], acts[i])[0] for i in range(init.shape[0])])
+    return synthetic_segments(simulate, ground_truth(prior, seed=1), sim, n_seg=3, horizon=4, seed=2), prior
+
+
+@pytest.mark.parametrize("motor,cost_mode", [(3, 0), (0, 1), (1, 0), (2, 0)])
+def test_plain_matches_oracle_fp64(model, sim, segs_small, motor, cost_mode):
+    import dataclasses
+    import spio
+    from spi_amd.baselines.plain_torch import PlainTorchRollout
+    segs, prior = segs_small
+    sim = dataclasses.replace(sim, motor_model=motor)
+    rng = np.random.default_rng(motor)
+    rows = np.repeat(prior.mean[None], 4, axis=0)
+    rows[:, prior.free] += rng.normal(size=(4, int(prior.free.sum()))) * prior.std[prior.free]
+    rows = np.clip(rows, prior.lo, prior.hi)
+    if motor in (1, 2):      # gain-type motor models: unit-order gains keep the robot on its feet
+        rows[:, 22:25] = rng.uniform(0.8, 1.2, size=(4, 3))
+    w = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05]) if cost_mode == 0 else np.array([10.0, 5.0, 0, 0, 1.0, 0])
+    ref = spio.Oracle(model, sim).rollout_cost(rows, segs.init, segs.actions, segs.targets, cost_mode, w)
+    f = lambda x: torch.tensor(np.asarray(x), dtype=torch.float64)
+    pl = PlainTorchRollout(sim, model, cost_mode=cost_mode, weights=w, device="cpu", dtype=torch.float64)
+    got = pl.segment_costs(f(rows.T), f(segs.init), f(segs.actions), f(segs.targets)).numpy().T
+    # same model in float64; the two differ only by the fixed-joint folding and summation order
+    assert np.allclose(got, ref, rtol=1e-9, atol=1e-12), np.abs(got - ref).max()
