@@ -45,6 +45,10 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-candidates", type=int, default=256)
     p.add_argument("--no-plain-baseline", action="store_true")
+    p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "serve"],
+                   help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
+                        "64 batched uneven problems, Fisher scoring of excitations and the serving loop")
+    p.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
     p.add_argument("--profile-launch", action="store_true",
                    help="run one rollout_cost launch of the workload and a 256 MiB device copy (for rocprofv3 "
                         "--pmc FETCH_SIZE / WRITE_SIZE calibration), then exit")
@@ -59,6 +63,16 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.workload != "cem22":
+        if not torch.cuda.is_available():
+            raise SystemExit("bench.py needs a GPU")
+        from spi_amd import workloads
+        if args.workload == "serve":
+            out = workloads.serve(n_requests=args.requests)
+        else:
+            out = getattr(workloads, args.workload)(args.steps, args.warmup)
+        print(json.dumps(out))
+        return
     from spi_amd.agents.cem import CEMConfig, DeviceCEM
     from spi_amd.agents.sysid import SysIdProblem, ground_truth, identification_prior
     from spi_amd.data import synthetic_segments
@@ -171,13 +185,12 @@ def main():
     print(json.dumps(out))
 
 
-def plain_gpu_baseline(model, sim, segs, prior, weights, steps=2):
+def plain_gpu_baseline(model, sim, segs, prior, weights):
     """The same rollouts as PyTorch eager ops (Python loop over control steps, decimation and
-    sub-intervals; spi_amd.baselines.plain_torch) on all 4096 x 256 lanes of one CEM iteration,
-    timed over ``steps`` control steps."""
+    sub-intervals; spi_amd.baselines.plain_torch), timed over one control step for a few lane
+    counts and both ways of forming the 6x6 products; the best rate is reported."""
     import torch
     from spi_amd.baselines.plain_torch import PlainTorchRollout
-    pl = PlainTorchRollout(sim, model, weights=weights, device="cuda")
     rng = np.random.default_rng(0)
     rows = np.repeat(prior.mean[None], N_SAMPLES, axis=0)
     rows[:, prior.free] += rng.normal(size=(N_SAMPLES, int(prior.free.sum()))) * prior.std[prior.free]
@@ -185,15 +198,24 @@ def plain_gpu_baseline(model, sim, segs, prior, weights, steps=2):
                          device="cuda")
     f = lambda x: torch.tensor(x, dtype=torch.float32, device="cuda")
     init, acts, tgts = f(segs.init), f(segs.actions), f(segs.targets)
-    pl.segment_costs(theta[:, :64], init[:8], acts[:8], tgts[:8], steps=1)          # warm
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    pl.segment_costs(theta, init, acts, tgts, steps=steps)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t
-    n = N_SAMPLES * segs.n_seg * steps
-    return {"value": round(n / dt, 1), "unit": "env-steps/s",
-            "sample": f"{N_SAMPLES} candidates x {segs.n_seg} segments x {steps} control steps ({dt:.2f} s)"}
+    tried = {}
+    for products in ("bcast", "bmm"):
+        pl = PlainTorchRollout(sim, model, weights=weights, device="cuda", products=products)
+        pl.segment_costs(theta[:, :64], init[:2], acts[:2], tgts[:2], steps=1)          # warm
+        for n_seg in (4, 32, 256):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            pl.segment_costs(theta, init[:n_seg], acts[:n_seg], tgts[:n_seg], steps=1)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            tried[f"{products}_{N_SAMPLES * n_seg}_lanes"] = round(N_SAMPLES * n_seg / dt, 1)
+            if dt > 8.0:
+                break
+        del pl
+        torch.cuda.empty_cache()
+    best = max(tried, key=tried.get)
+    return {"value": tried[best], "unit": "env-steps/s", "best": best, "tried": tried,
+            "sample": "1 control step (4 physics steps x 4 sub-intervals) per configuration"}
 
 
 def cpu_baseline(model, sim, segs, prior, n_cand):

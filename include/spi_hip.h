@@ -27,6 +27,7 @@ extern "C" {
 #define SPI_NUM_DOF 12
 #define SPI_ROOT_DIM 13
 #define SPI_STATE_DIM 37 /* root(13) + q(12) + qd(12) */
+#define SPI_MAX_TANGENTS 32
 #define SPI_NUM_FEET 4
 #define SPI_NUM_ISAAC_BODIES 19
 #define SPI_P_COUNT 25
@@ -104,6 +105,13 @@ typedef struct SpiBatchProblem {
   float* best;                  /* [SPI_P_COUNT] */
   float* best_cost;             /* [1] */
 } SpiBatchProblem;
+
+/* Tangent directions of a sensitivity rollout: slot k seeds parameter row param[k] of the
+ * SoA parameter array (0 <= param[k] < SPI_P_COUNT) with a unit tangent. */
+typedef struct SpiTangentSpec {
+  int n_tan;                 /* 1 .. SPI_MAX_TANGENTS */
+  int param[SPI_P_COUNT];
+} SpiTangentSpec;
 
 const char* spi_last_error(void);
 int spi_abi_version(void);
@@ -222,6 +230,45 @@ int spi_batch_rollout_cost(const SpiSimArgs* sim, const SpiBatchProblem* probs_d
 /* Per problem (one workgroup each): cost_out = weighted sum over segments (fixed order), then
  * the CEM update of spi_cem_update, then iteration += 1. */
 int spi_batch_cem_update(const SpiBatchProblem* probs_dev, int n_prob, hipStream_t stream);
+
+/* ---- differentiable rollout and Fisher information (active exploration) ---- */
+
+/* Trajectory sensitivities by forward (tangent) propagation through the exact discrete
+ * rollout.  Group g = e * n_samples + c rolls out parameter sample c (params_dev[k * ld + c])
+ * under excitation e (actions_dev [n_exc][horizon][12]) from init_dev [37]; after every
+ * control step t it records y_t = [root (13, IsaacGym layout) | q (12)] -- the state the
+ * reference's FIM reward differentiates (active_sysid_openloop.py:402-426) -- and
+ * J_dev[g][t * 25 + i][k] = d y_t[i] / d theta[tan->param[k]].  y_dev [G][horizon][25] may be
+ * NULL.  status_dev [G]: 0 ok, 1 the state left the finite / state_limit range (its
+ * remaining rows of J are zero).  Replaces the (P + 1)-env finite-difference groups of
+ * active_sysid_openloop.py:29-80 with exact derivatives. */
+int spi_traj_sensitivity(const SpiSimArgs* sim, const SpiTangentSpec* tan, const float* params_dev, int ld,
+                         int n_samples, const float* init_dev, const float* actions_dev, int n_exc, int horizon,
+                         float* J_dev, float* y_dev, int* status_dev, hipStream_t stream);
+
+/* Fisher information of every group, F_dev[g] = J_g^T J_g ([n_tan][n_tan], n_tan <= 32),
+ * with J_g = J_dev[g] of n_rows x n_tan, accumulated with FP32 MFMA (32x32x2). */
+int spi_fim_assemble(const float* J_dev, long long n_groups, int n_rows, int n_tan, float* F_dev,
+                     hipStream_t stream);
+
+/* Per set s of set_size consecutive groups (one excitation): mean of the groups' F (FP64,
+ * fixed order, groups with status != 0 skipped), plus reg * I, FP64 Cholesky, logdet.
+ * logdet_dev [n_sets] (-inf when not positive definite or no valid group), n_valid_dev
+ * [n_sets], F_mean_dev [n_sets][n_tan][n_tan] (may be NULL). */
+int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan, double reg,
+                   double* F_mean_dev, double* logdet_dev, int* n_valid_dev, hipStream_t stream);
+
+/* Tracking cost and its exact gradient for every (segment, candidate) pair (layouts as
+ * spi_rollout_cost): cost_dev [n_seg][n_cand], grad_dev [n_seg][n_cand][n_tan] =
+ * d cost / d theta[tan->param[k]] (+inf cost and zero gradient for diverged lanes). */
+int spi_rollout_cost_grad(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
+                          const float* params_dev, int ld, int n_cand, const float* segments_dev, int n_seg,
+                          int horizon, float* cost_dev, float* grad_dev, hipStream_t stream);
+
+/* Excitations as Fourier series: actions_dev[e][t][j] = scale * sum_k coeff_dev[e][j][k] *
+ * sin(2 pi (k + 1) t dt_ctrl / period), k < n_terms. */
+int spi_fourier_excitations(const float* coeff_dev, int n_exc, int n_terms, int horizon, float dt_ctrl, float period,
+                            float scale, float* actions_dev, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,5 +1,6 @@
 // ORACLE — test infrastructure only (see spi_oracle.hpp).  C entry points for ctypes.
 #include "spi_oracle.hpp"
+#include "spi_dual64.hpp"
 #include "spi_philox.hpp"
 
 #include <cstdlib>
@@ -253,6 +254,79 @@ int spio_rollout_cost_f32(void* hp, const float* par, int n_cand, const float* s
     }
   }
   return 0;
+}
+
+// ---------------------------------------------------------------- FP64 derivatives
+// Forward-mode derivatives of the FP64 engine with respect to P <= 25 parameter rows idx[].
+using DualP = spio_ad::DualD<P_COUNT>;
+
+static void dual_engine(const Handle<double>* h, Engine<DualP>& eng) {
+  eng.tree.build(h->eng.tree.links.data(), h->eng.tree.nl, h->eng.tree.contacts.data(),
+                 int(h->eng.tree.contacts.size()));
+  eng.cfg = h->eng.cfg;
+  eng.base_link = h->eng.base_link;
+}
+static void dual_params(const double* par, const int* idx, int P, DualP* p) {
+  for (int k = 0; k < P_COUNT; ++k) p[k] = DualP(par[k]);
+  for (int m = 0; m < P; ++m) p[idx[m]].d[m] = 1.0;
+}
+
+// Trajectory and sensitivities: after control step t, y = [root (13) | q (12)] and
+// J[t][i][m] = d y[i] / d par[idx[m]].  y_out [T][25], J_out [T][25][P].
+int spio_traj_sens(void* hp, const double* par, const int* idx, int P, const float* init37, const float* actions,
+                   int n_steps, double* y_out, double* J_out) {
+  if (P < 1 || P > P_COUNT) return -1;
+  auto* h = static_cast<Handle<double>*>(hp);
+  Engine<DualP> eng;
+  dual_engine(h, eng);
+  DualP p[P_COUNT];
+  dual_params(par, idx, P, p);
+  eng.apply_base_params(p);
+  State<DualP> st;
+  load37(st, init37);
+  for (int t = 0; t < n_steps; ++t) {
+    DualP a[12];
+    for (int j = 0; j < 12; ++j) a[j] = DualP(double(actions[12 * t + j]));
+    eng.control_step(st, a, p);
+    for (int i = 0; i < 25; ++i) {
+      const DualP& y = i < 13 ? st.root[i] : st.q[i - 13];
+      y_out[t * 25 + i] = y.v;
+      for (int m = 0; m < P; ++m) J_out[(size_t(t) * 25 + i) * P + m] = y.d[m];
+    }
+  }
+  return finite_state(st) ? 0 : 1;
+}
+
+// Cost of one parameter vector on n_seg segments and its gradient with respect to idx[]:
+// cost_out [n_seg], grad_out [n_seg][P].
+int spio_cost_grad(void* hp, const double* par, const int* idx, int P, const float* seg_init, const float* seg_actions,
+                   const float* seg_targets, int n_seg, int H, int cost_mode, const double* w, double* cost_out,
+                   double* grad_out) {
+  if (P < 1 || P > P_COUNT) return -1;
+  auto* h = static_cast<Handle<double>*>(hp);
+  Engine<DualP> eng;
+  dual_engine(h, eng);
+  DualP p[P_COUNT];
+  dual_params(par, idx, P, p);
+  eng.apply_base_params(p);
+  int bad = 0;
+  for (int s = 0; s < n_seg; ++s) {
+    State<DualP> st;
+    load37(st, seg_init + 37 * s);
+    DualP acc(0.0);
+    for (int t = 0; t < H; ++t) {
+      DualP a[12];
+      for (int j = 0; j < 12; ++j) a[j] = DualP(double(seg_actions[(size_t(s) * H + t) * 12 + j]));
+      eng.control_step(st, a, p);
+      const float* tg = seg_targets + (size_t(s) * H + t) * 37;
+      if (cost_mode == COST_TRACKING || t == H - 1) acc += step_cost<DualP>(cost_mode, w, st, tg);
+    }
+    const bool ok = finite_state(st) && std::isfinite(acc.v);
+    cost_out[s] = ok ? acc.v : INFINITY;
+    for (int m = 0; m < P; ++m) grad_out[size_t(s) * P + m] = ok ? acc.d[m] : 0.0;
+    bad += !ok;
+  }
+  return bad;
 }
 
 // ---------------------------------------------------------------- RNG and CEM restatement

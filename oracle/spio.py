@@ -73,6 +73,9 @@ def lib():
         L.spio_control_torque.argtypes = [C.POINTER(SimConfig), dp, dp, dp, dp, dp]
         L.spio_simulate.argtypes = [vp, dp, fp, fp, C.c_int, fp]
         L.spio_rollout_cost.argtypes = [vp, dp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp]
+        ipp = C.POINTER(C.c_int)
+        L.spio_traj_sens.argtypes = [vp, dp, ipp, C.c_int, fp, fp, C.c_int, dp, dp]
+        L.spio_cost_grad.argtypes = [vp, dp, ipp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.spio_rollout_cost_f32.argtypes = [vp, fp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, fp,
                                             C.c_int]
         L.spio_max_threads.restype = C.c_int
@@ -276,6 +279,33 @@ class Oracle:
         lib().spio_rollout_cost(self.h, _dp(par), par.shape[0], _fp(seg_init), _fp(seg_actions), _fp(seg_targets),
                                 n_seg, H, cost_mode, _dp(w), _dp(out))
         return out
+
+    def traj_sens(self, par, idx, init37, actions):
+        """FP64 forward-mode sensitivities: y [T, 25] (root 13 + q 12 after each control step)
+        and J [T, 25, P] = d y / d par[idx]."""
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        init37 = np.ascontiguousarray(init37, dtype=np.float32)
+        actions = np.ascontiguousarray(actions, dtype=np.float32)
+        T, P = actions.shape[0], idx.shape[0]
+        y, J = np.zeros((T, 25)), np.zeros((T, 25, P))
+        rc = lib().spio_traj_sens(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), P, _fp(init37),
+                                  _fp(actions), T, _dp(y), _dp(J))
+        return y, J, rc
+
+    def cost_grad(self, par, idx, seg_init, seg_actions, seg_targets, cost_mode, weights):
+        """FP64 cost of one parameter vector per segment [n_seg] and its gradient [n_seg, P]."""
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        seg_init = np.ascontiguousarray(seg_init, dtype=np.float32)
+        seg_actions = np.ascontiguousarray(seg_actions, dtype=np.float32)
+        seg_targets = np.ascontiguousarray(seg_targets, dtype=np.float32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        n_seg, H = seg_actions.shape[0], seg_actions.shape[1]
+        cost, grad = np.zeros(n_seg), np.zeros((n_seg, idx.shape[0]))
+        lib().spio_cost_grad(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), idx.shape[0], _fp(seg_init),
+                             _fp(seg_actions), _fp(seg_targets), n_seg, H, cost_mode, _dp(w), _dp(cost), _dp(grad))
+        return cost, grad
 
     def rollout_cost_f32(self, par, seg_init, seg_actions, seg_targets, cost_mode, weights, n_threads=0):
         assert self.f32

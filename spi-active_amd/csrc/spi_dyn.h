@@ -22,8 +22,13 @@
 #ifndef SPI_REAL_T
 #define SPI_REAL_T float
 #endif
+// A second instantiation in another translation unit (e.g. tangent arithmetic) puts the
+// code in its own namespace so the two never share a symbol.
+#ifndef SPI_DYN_NS
+#define SPI_DYN_NS spi
+#endif
 
-namespace spi {
+namespace SPI_DYN_NS {
 
 using Real = SPI_REAL_T;
 
@@ -219,6 +224,8 @@ SPI_HD void abi_add(ABI& a, const ABI& b) {
 }
 
 // ------------------------------------------------------------------ scalar math
+// (a scalar type that brings its own versions defines SPI_REAL_HAS_MATH)
+#ifndef SPI_REAL_HAS_MATH
 // tanh: odd Taylor series near 0 (no cancellation), 1 - 2/(1+e^{2x}) elsewhere.
 SPI_HD Real spi_tanh(Real x) {
   Real ax = fabsf(x);
@@ -257,6 +264,8 @@ SPI_HD Real spi_rsqrt(Real x) {
   return 1.0f / sqrtf(x);
 #endif
 }
+
+#endif  // SPI_REAL_HAS_MATH
 
 // ------------------------------------------------------------------ state and parameters
 struct State {
@@ -671,4 +680,4 @@ template <class TgtT> SPI_HD Real step_cost(const SpiCostArgs& c, const State& s
   return c.w[0] * sqrtf(e0) + c.w[1] * sqrtf(e1) + c.w[4] * sqrtf(e4);
 }
 
-}  // namespace spi
+}  // namespace SPI_DYN_NS

@@ -1,0 +1,169 @@
+// Fisher information of trajectory sensitivities and its log-determinant (gfx950).
+//
+// * Assembly: F = J^T J per (parameter sample, excitation) group with the 32x32x2 FP32 matrix
+//   instruction, one wavefront per group -- the only matrix-core work of the engine.
+// * logdet: FP64 mean over the parameter samples of one excitation, FP64 Cholesky.
+// * Fourier excitations: the candidate command sequences scored by the objective.
+// Strict IEEE semantics (no finite-math): -inf marks a matrix that is not positive definite.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+
+#include "spi_types.h"
+
+namespace {
+
+int ffail(int code, const char* msg) {
+  spi_set_error(msg);
+  return code;
+}
+int flaunch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return SPI_OK;
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%s: %s", what, hipGetErrorString(e));
+  spi_set_error(buf);
+  return SPI_ERR_HIP;
+}
+
+// F_g = J_g^T J_g: one wavefront per group.  Lane l supplies A[l % 32][l / 32] = B[l / 32][l % 32]
+// = J[k0 + l / 32][l % 32] (zero beyond n_tan), so each instruction adds the outer products
+// of two rows of J.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+__global__ __launch_bounds__(256) void fim_assemble_kernel(const float* __restrict__ J, long long G, int K, int P,
+                                                           float* __restrict__ F) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long g = static_cast<long long>(blockIdx.x) * 4 + wave;
+  if (g >= G) return;   // uniform per wavefront
+  const float* Jg = J + static_cast<size_t>(g) * K * P;
+  const int col = lane & 31, half = lane >> 5;
+  const bool live = col < P;
+  floatx16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  int k0 = 0;
+  for (; k0 + 8 <= K; k0 += 8) {
+    float v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = live ? Jg[static_cast<size_t>(k0 + 2 * u + half) * P + col] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v[u], v[u], acc, 0, 0, 0);
+  }
+  for (; k0 < K; k0 += 2) {
+    const int k = k0 + half;
+    const float v = (live && k < K) ? Jg[static_cast<size_t>(k) * P + col] : 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
+  }
+  // result layout: D[i][j], j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
+  float* Fg = F + static_cast<size_t>(g) * P * P;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = 8 * (r >> 2) + 4 * half + (r & 3);
+    if (i < P && col < P) Fg[i * P + col] = acc[r];
+  }
+}
+
+// One workgroup per set: FP64 mean of the valid groups' F, + reg I, Cholesky, logdet.
+__global__ __launch_bounds__(256) void fim_logdet_kernel(const float* __restrict__ F, const int* __restrict__ status,
+                                                         int set_size, int P, double reg, double* __restrict__ Fmean,
+                                                         double* __restrict__ logdet, int* __restrict__ n_valid) {
+  __shared__ double M[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
+  __shared__ int cnt;
+  const int s = blockIdx.x;
+  const size_t g0 = static_cast<size_t>(s) * set_size;
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int c = 0; c < set_size; ++c) n += status[g0 + c] == 0;
+    cnt = n;
+  }
+  __syncthreads();
+  const int n = cnt;
+  for (int e = threadIdx.x; e < P * P; e += blockDim.x) {
+    double acc = 0.0;
+    for (int c = 0; c < set_size; ++c)
+      if (status[g0 + c] == 0) acc += static_cast<double>(F[(g0 + c) * P * P + e]);
+    const int i = e / P, j = e % P;
+    double m = n > 0 ? acc / n : 0.0;
+    if (i == j) m += reg;
+    M[e] = m;
+  }
+  __syncthreads();
+  if (Fmean)
+    for (int e = threadIdx.x; e < P * P; e += blockDim.x) Fmean[static_cast<size_t>(s) * P * P + e] = M[e];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // symmetric part, then in-place Cholesky (lower) in FP64
+    double ld = 0.0;
+    bool pd = n > 0;
+    for (int j = 0; j < P && pd; ++j) {
+      double d = M[j * P + j];
+      for (int k = 0; k < j; ++k) d -= M[j * P + k] * M[j * P + k];
+      if (!(d > 0.0)) { pd = false; break; }
+      const double ljj = sqrt(d);
+      M[j * P + j] = ljj;
+      ld += 2.0 * log(ljj);
+      for (int i = j + 1; i < P; ++i) {
+        double v = 0.5 * (M[i * P + j] + M[j * P + i]);
+        for (int k = 0; k < j; ++k) v -= M[i * P + k] * M[j * P + k];
+        M[i * P + j] = v / ljj;
+      }
+    }
+    logdet[s] = pd ? ld : -INFINITY;
+    n_valid[s] = n;
+  }
+}
+
+__global__ void fourier_kernel(const float* __restrict__ coeff, int n_exc, int n_terms, int H, float dt, float period,
+                               float scale, float* __restrict__ out) {
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long long total = static_cast<long long>(n_exc) * H * 12;
+  if (i >= total) return;
+  const int j = static_cast<int>(i % 12), t = static_cast<int>((i / 12) % H), e = static_cast<int>(i / (12LL * H));
+  const float* c = coeff + (static_cast<size_t>(e) * 12 + j) * n_terms;
+  const double w = 2.0 * M_PI * t * static_cast<double>(dt) / period;
+  double acc = 0.0;
+  for (int k = 0; k < n_terms; ++k) acc += static_cast<double>(c[k]) * sin((k + 1) * w);
+  out[i] = static_cast<float>(scale * acc);
+}
+
+}  // namespace
+
+extern "C" {
+
+int spi_fim_assemble(const float* J_dev, long long n_groups, int n_rows, int n_tan, float* F_dev,
+                     hipStream_t stream) {
+  if (n_groups < 0 || n_rows < 1 || n_tan < 1 || n_tan > SPI_MAX_TANGENTS || !J_dev || !F_dev)
+    return ffail(SPI_ERR_INVALID_ARG, "spi_fim_assemble: bad arguments");
+  if (n_groups == 0) return SPI_OK;
+  const long long blocks = (n_groups + 3) / 4;
+  if (blocks > 0x7fffffffLL) return ffail(SPI_ERR_TOO_LARGE, "spi_fim_assemble: too many groups");
+  fim_assemble_kernel<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(J_dev, n_groups, n_rows, n_tan, F_dev);
+  return flaunch("fim_assemble_kernel");
+}
+
+int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan, double reg,
+                   double* F_mean_dev, double* logdet_dev, int* n_valid_dev, hipStream_t stream) {
+  if (n_sets < 0 || set_size < 1 || n_tan < 1 || n_tan > SPI_MAX_TANGENTS || !(reg >= 0.0) || !F_dev ||
+      !status_dev || !logdet_dev || !n_valid_dev)
+    return ffail(SPI_ERR_INVALID_ARG, "spi_fim_logdet: bad arguments");
+  if (n_sets == 0) return SPI_OK;
+  fim_logdet_kernel<<<n_sets, 256, 0, stream>>>(F_dev, status_dev, set_size, n_tan, reg, F_mean_dev, logdet_dev,
+                                                n_valid_dev);
+  return flaunch("fim_logdet_kernel");
+}
+
+int spi_fourier_excitations(const float* coeff_dev, int n_exc, int n_terms, int horizon, float dt_ctrl, float period,
+                            float scale, float* actions_dev, hipStream_t stream) {
+  if (n_exc < 0 || n_terms < 0 || horizon < 0 || !(dt_ctrl > 0.0f) || !(period > 0.0f) || !coeff_dev || !actions_dev)
+    return ffail(SPI_ERR_INVALID_ARG, "spi_fourier_excitations: bad arguments");
+  const long long total = static_cast<long long>(n_exc) * horizon * 12;
+  if (total == 0) return SPI_OK;
+  const long long blocks = (total + 255) / 256;
+  if (blocks > 0x7fffffffLL) return ffail(SPI_ERR_TOO_LARGE, "spi_fourier_excitations: too large");
+  fourier_kernel<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(coeff_dev, n_exc, n_terms, horizon, dt_ctrl,
+                                                                    period, scale, actions_dev);
+  return flaunch("fourier_kernel");
+}
+
+}  // extern "C"

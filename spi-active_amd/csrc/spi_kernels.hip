@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "spi_dyn.h"
+#include "spi_launch.h"
 
 using namespace spi;
 
@@ -28,16 +29,7 @@ int hip_check(hipError_t e, const char* what) {
 }
 int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
 
-// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  Remap the
-// hardware block id so that logically consecutive blocks (same segment, neighbouring
-// candidate blocks) run on the same XCD and share its L2.
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  constexpr int NX = 8;
-  int q = nb / NX, r = nb % NX;
-  int x = b % NX, i = b / NX;
-  int base = x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q;
-  return base + i;
-}
+using spi_launch::xcd_remap;
 
 __device__ __forceinline__ LaneParams load_params(const float* __restrict__ par, int ld, int i) {
   LaneParams lp;
@@ -54,9 +46,7 @@ __device__ __forceinline__ LaneParams load_params(const float* __restrict__ par,
   return lp;
 }
 
-// Finiteness from the exponent bits: stays correct when the file is compiled with
-// -ffinite-math-only (which lets the compiler fold the model's many x*0 terms).
-__device__ __forceinline__ bool finite_bits(float x) { return (__float_as_uint(x) & 0x7f800000u) != 0x7f800000u; }
+using spi_launch::finite_bits;
 
 constexpr int kRolloutThreads = 256;
 

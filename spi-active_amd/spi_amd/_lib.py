@@ -19,6 +19,7 @@ from .params import SimParams
 SPI_P_COUNT = 25
 SPI_STATE_DIM = 37
 SPI_MAX_CEM_SAMPLES = 8192
+SPI_MAX_TANGENTS = 32
 
 
 class SpiSimArgs(C.Structure):
@@ -31,6 +32,23 @@ class SpiSimArgs(C.Structure):
 
 class SpiCostArgs(C.Structure):
     _fields_ = [("mode", C.c_int), ("w", C.c_float * 6)]
+
+
+class SpiTangentSpec(C.Structure):
+    _fields_ = [("n_tan", C.c_int), ("param", C.c_int * SPI_P_COUNT)]
+
+
+def tangent_spec(param_rows) -> SpiTangentSpec:
+    rows = [int(r) for r in param_rows]
+    if not (1 <= len(rows) <= min(SPI_MAX_TANGENTS, SPI_P_COUNT)):
+        raise ValueError(f"1..{SPI_P_COUNT} tangent directions, got {len(rows)}")
+    t = SpiTangentSpec()
+    t.n_tan = len(rows)
+    for k, r in enumerate(rows):
+        if not 0 <= r < SPI_P_COUNT:
+            raise ValueError(f"parameter row {r} out of range")
+        t.param[k] = r
+    return t
 
 
 def sim_args(sim: SimParams, state_limit: float = 1.0e4) -> SpiSimArgs:
@@ -95,6 +113,11 @@ def lib() -> C.CDLL:
         "spi_batch_cem_sample": [vp, ip, ip, vp],
         "spi_batch_rollout_cost": [sa, vp, vp, ip, C.c_longlong, ip, ip, ip, ip, vp, vp],
         "spi_batch_cem_update": [vp, ip, vp],
+        "spi_traj_sensitivity": [sa, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp, vp],
+        "spi_fim_assemble": [vp, C.c_longlong, ip, ip, vp, vp],
+        "spi_fim_logdet": [vp, vp, ip, ip, ip, C.c_double, vp, vp, vp, vp],
+        "spi_rollout_cost_grad": [sa, ca, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, ip, ip, vp, vp, vp],
+        "spi_fourier_excitations": [vp, ip, ip, ip, fp, fp, fp, vp, vp],
     }
     for name, args in sigs.items():
         fn = getattr(L, name)

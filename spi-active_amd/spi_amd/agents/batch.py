@@ -48,6 +48,26 @@ class ProblemSpec:
     seg_weights: Optional[torch.Tensor] = None
 
 
+def _validate(k: int, s: ProblemSpec) -> None:
+    """The batched kernels read sizes from the device table and cannot check them: check here."""
+    from ..data import segment_stride
+    if not (1 <= s.n_samples <= _lib.SPI_MAX_CEM_SAMPLES):
+        raise ValueError(f"problem {k}: n_samples {s.n_samples} outside [1, {_lib.SPI_MAX_CEM_SAMPLES}]")
+    n_elite = max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9)))
+    if n_elite > s.n_samples:
+        raise ValueError(f"problem {k}: elite_frac {s.elite_frac} gives more elites than samples")
+    if not (0.0 <= s.alpha < 1.0):
+        raise ValueError(f"problem {k}: alpha {s.alpha} outside [0, 1)")
+    if s.horizon < 1 or s.segments.dim() != 2 or s.segments.shape[0] < 1 or \
+            s.segments.shape[1] != segment_stride(s.horizon):
+        raise ValueError(f"problem {k}: segments {tuple(s.segments.shape)} do not match horizon {s.horizon}")
+    if s.segments.dtype != torch.float32 or not s.segments.is_contiguous() or s.segments.device.type != "cuda":
+        raise ValueError(f"problem {k}: segments must be a contiguous float32 device tensor")
+    if s.seg_weights is not None and (s.seg_weights.numel() != s.segments.shape[0] or
+                                      s.seg_weights.dtype != torch.float32 or s.seg_weights.device.type != "cuda"):
+        raise ValueError(f"problem {k}: seg_weights must be float32 [n_seg] on the device")
+
+
 class _Buffers:
     """Device state of one problem (CEM moments, samples, costs, best so far)."""
     def __init__(self, spec: ProblemSpec, dev):
@@ -78,6 +98,8 @@ class BatchCEM:
                  iterations_done: Optional[Sequence[int]] = None):
         if not specs:
             raise ValueError("no problems")
+        for k, s in enumerate(specs):
+            _validate(k, s)
         self.engine = engine
         self.dev = engine.device
         # longest horizon first: the persistent grid takes tickets in table order
@@ -116,8 +138,17 @@ class BatchCEM:
         self.prefix = torch.tensor(prefix, dtype=torch.int64, device=self.dev)
         self.n_tasks = tot
 
+    def device_iterations(self) -> List[int]:
+        """Iteration counters as the device table holds them (table order)."""
+        raw = self.table.cpu().numpy().tobytes()
+        arr = (SpiBatchProblem * len(self.specs)).from_buffer_copy(raw)
+        return [int(p.iteration) for p in arr]
+
     def set_launch(self, block: int, unroll: int) -> None:
+        """Change envs per block / unroll depth.  Graphs captured before this call are stale."""
         if block != self.block:
+            torch.cuda.current_stream().synchronize()
+            self.start_iter = self.device_iterations()     # keep each problem's Philox position
             self.block = block
             self._build_table()
         self.unroll = unroll

@@ -32,10 +32,10 @@ def _skew(v: torch.Tensor) -> torch.Tensor:
                         torch.stack([-v[..., 1], v[..., 0], z], -1)], -2)
 
 
-def _xform(E: torch.Tensor, r: torch.Tensor) -> torch.Tensor:
+def _xform(E: torch.Tensor, r: torch.Tensor, mm) -> torch.Tensor:
     """Plucker motion transform [E 0; -E r^ E] (angular part first)."""
     top = torch.cat([E, torch.zeros_like(E)], -1)
-    bot = torch.cat([-E @ _skew(r), E], -1)
+    bot = torch.cat([-mm(E, _skew(r).expand(E.shape)), E], -1)
     return torch.cat([top, bot], -2)
 
 
@@ -50,6 +50,19 @@ def _crf(v: torch.Tensor) -> torch.Tensor:
 
 def _mv(M: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
     return (M @ v.unsqueeze(-1)).squeeze(-1)
+
+
+def _mm(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    return A @ B
+
+
+# the same products as broadcast multiply + sum (elementwise kernels, no batched GEMM calls)
+def _mv_bcast(M: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
+    return (M * v.unsqueeze(-2)).sum(-1)
+
+
+def _mm_bcast(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    return (A.unsqueeze(-1) * B.unsqueeze(-3)).sum(-2)
 
 
 def _spatial_inertia(m, h, I):
@@ -110,8 +123,12 @@ class PlainTorchRollout:
 
     def __init__(self, sim: SimParams, model: Optional[Dict] = None, cost_mode: int = _COST_TRACKING,
                  weights=(10.0, 5.0, 1.0, 1.0, 1.0, 0.05), device="cuda", dtype=torch.float32,
-                 state_limit: float = 1.0e4):
+                 state_limit: float = 1.0e4, products: str = "bmm"):
+        """products: "bmm" (batched matrix products) or "bcast" (broadcast multiply + sum)."""
         model = model or load_model()
+        if products not in ("bmm", "bcast"):
+            raise ValueError(products)
+        self._mv, self._mm = (_mv, _mm) if products == "bmm" else (_mv_bcast, _mm_bcast)
         self.sim, self.dev, self.dt_ = sim, torch.device(device), dtype
         self.cost_mode, self.w = int(cost_mode), [float(x) for x in weights]
         self.state_limit = state_limit
@@ -183,35 +200,35 @@ class PlainTorchRollout:
         vp = v0[:, None, :].expand(N, 4, 6)
         for j in range(3):
             Rj = _axis_rotation(self.axes[j], cq[..., j], sq[..., j])       # child -> parent
-            Xj = _xform(Rj.transpose(-1, -2), self.r[j])
-            vj = _mv(Xj, vp) + self.S[j] * st["qd"][..., j, None]
-            c.append(_mv(_crm(vj), self.S[j] * st["qd"][..., j, None]))
+            Xj = _xform(Rj.transpose(-1, -2), self.r[j], self._mm)
+            vj = self._mv(Xj, vp) + self.S[j] * st["qd"][..., j, None]
+            c.append(self._mv(_crm(vj), self.S[j] * st["qd"][..., j, None]))
             X.append(Xj)
             v.append(vj)
             E.append(Rj)
             vp = vj
         # calf pose and foot contact (world frame z up, plane z = 0)
         Rb = R[:, None]
-        pc = self.r[0] + _mv(E[0], self.r[1] + _mv(E[1], self.r[2]))      # calf origin, base coords
-        pcw = st["p"][:, None] + _mv(Rb, pc)
-        Rc = Rb @ E[0] @ E[1] @ E[2]
-        centre = pcw + _mv(Rc, self.foot)
+        pc = self.r[0] + self._mv(E[0], self.r[1] + self._mv(E[1], self.r[2]))      # calf origin, base coords
+        pcw = st["p"][:, None] + self._mv(Rb, pc)
+        Rc = self._mm(self._mm(self._mm(Rb.expand(E[0].shape), E[0]), E[1]), E[2])
+        centre = pcw + self._mv(Rc, self.foot)
         pen = self.rad - centre[..., 2]
         down = -self.rad[:, None] * Rc[..., 2, :]                          # Rc^T (0, 0, -rad)
         pcl = self.foot + down
         vl = v[2][..., 3:] + torch.cross(v[2][..., :3], pcl, dim=-1)
-        vw = _mv(Rc, vl)
+        vw = self._mv(Rc, vl)
         fn = torch.clamp(s.k_contact * pen - s.c_contact * vw[..., 2], min=0.0)
         vt = torch.sqrt(vw[..., 0] ** 2 + vw[..., 1] ** 2)
         ft = torch.minimum(s.c_tangent * vt, s.mu * fn)
         scale = torch.where(vt > 0, ft / torch.where(vt > 0, vt, torch.ones_like(vt)), torch.zeros_like(vt))
         fw = torch.stack([-scale * vw[..., 0], -scale * vw[..., 1], fn], -1)
         fw = torch.where((pen > 0)[..., None], fw, torch.zeros_like(fw))
-        fl = _mv(Rc.transpose(-1, -2), fw)
+        fl = self._mv(Rc.transpose(-1, -2), fw)
         fext = torch.cat([torch.cross(pcl, fl, dim=-1), fl], -1)
         # backward pass
         IA = self.I[2].expand(N, 4, 6, 6)
-        pA = _mv(_crf(v[2]), _mv(self.I[2], v[2])) - fext
+        pA = self._mv(_crf(v[2]), self._mv(self.I[2], v[2])) - fext
         U, D, u = [None] * 3, [None] * 3, [None] * 3
         for j in (2, 1, 0):
             k = self.axes[j]
@@ -219,26 +236,26 @@ class PlainTorchRollout:
             D[j] = U[j][..., k]
             u[j] = tau[..., j] - pA[..., k]
             Ia = IA - U[j][..., :, None] * U[j][..., None, :] / D[j][..., None, None]
-            pa = pA + _mv(Ia, c[j]) + U[j] * (u[j] / D[j])[..., None]
+            pa = pA + self._mv(Ia, c[j]) + U[j] * (u[j] / D[j])[..., None]
             XT = X[j].transpose(-1, -2)
-            IA_up, p_up = XT @ Ia @ X[j], _mv(XT, pa)
+            IA_up, p_up = self._mm(self._mm(XT, Ia), X[j]), self._mv(XT, pa)
             if j > 0:
                 IA = self.I[j - 1] + IA_up
-                pA = _mv(_crf(v[j - 1]), _mv(self.I[j - 1], v[j - 1])) + p_up
+                pA = self._mv(_crf(v[j - 1]), self._mv(self.I[j - 1], v[j - 1])) + p_up
             else:
                 IA0 = lp["I0"] + IA_up.sum(1)
-                p0 = _mv(_crf(v0), _mv(lp["I0"], v0)) + p_up.sum(1)
+                p0 = self._mv(_crf(v0), self._mv(lp["I0"], v0)) + p_up.sum(1)
         a0 = _solve_spd6(IA0, -p0)                                   # gravity-free
         qdd = []
         ap = a0[:, None, :].expand(N, 4, 6)
         for j in range(3):
-            aj = _mv(X[j], ap) + c[j]
+            aj = self._mv(X[j], ap) + c[j]
             x = (u[j] - (U[j] * aj).sum(-1)) / D[j]
             qdd.append(x)
             ap = aj + self.S[j] * x[..., None]
         g = torch.zeros_like(st["p"])
         g[:, 2] = -s.gravity
-        acc_v = a0[:, 3:] + _mv(R.transpose(-1, -2), g)
+        acc_v = a0[:, 3:] + self._mv(R.transpose(-1, -2), g)
         return a0[:, :3], acc_v, torch.stack(qdd, -1)
 
     def integrate(self, st, dw, dv, qdd, h):
@@ -247,7 +264,7 @@ class PlainTorchRollout:
         st["qd"] = st["qd"] + h * qdd
         st["q"] = st["q"] + h * st["qd"]
         R = _quat_to_rot(st["quat"])
-        st["p"] = st["p"] + h * _mv(R, st["v"])
+        st["p"] = st["p"] + h * self._mv(R, st["v"])
         hh = 0.5 * h
         t2 = hh * hh * (st["w"] ** 2).sum(-1)
         sfac = hh * (1.0 - t2 / 6.0)
@@ -266,13 +283,13 @@ class PlainTorchRollout:
     def state_from37(self, s37: torch.Tensor):
         R = _quat_to_rot(s37[:, 3:7])
         Rt = R.transpose(-1, -2)
-        return {"p": s37[:, 0:3].clone(), "quat": s37[:, 3:7].clone(), "v": _mv(Rt, s37[:, 7:10]),
-                "w": _mv(Rt, s37[:, 10:13]), "q": s37[:, 13:25].reshape(-1, 4, 3).clone(),
+        return {"p": s37[:, 0:3].clone(), "quat": s37[:, 3:7].clone(), "v": self._mv(Rt, s37[:, 7:10]),
+                "w": self._mv(Rt, s37[:, 10:13]), "q": s37[:, 13:25].reshape(-1, 4, 3).clone(),
                 "qd": s37[:, 25:37].reshape(-1, 4, 3).clone()}
 
     def state_to37(self, st) -> torch.Tensor:
         R = _quat_to_rot(st["quat"])
-        return torch.cat([st["p"], st["quat"], _mv(R, st["v"]), _mv(R, st["w"]), st["q"].reshape(-1, 12),
+        return torch.cat([st["p"], st["quat"], self._mv(R, st["v"]), self._mv(R, st["w"]), st["q"].reshape(-1, 12),
                           st["qd"].reshape(-1, 12)], -1)
 
     def control_step(self, lp, st, act):

@@ -6,8 +6,9 @@
   so a job that arrives while others are running joins at the next iteration boundary and
   leaves when its own iteration count is reached.  CEM state stays on the device for the
   whole life of a job; only membership changes rebuild the (small) problem table.
-* excitation-scoring jobs (FD Fisher information of candidate excitations,
-  agents/active.py) run on a second stream and overlap with the CEM iteration.
+* excitation-scoring jobs (logdet of the exact Fisher information of candidate excitations,
+  agents/fisher.py; or the reference's finite-difference trace, agents/active.py) run on a
+  second stream and overlap with the CEM iteration.
 
 A job's result does not depend on the other jobs in flight: each problem has its own Philox
 stream, iteration counter and buffers, and the batched kernels never mix problems.
@@ -24,6 +25,7 @@ import torch
 
 from .agents.active import FDFIMObjective
 from .agents.batch import BatchCEM, ProblemSpec, _Buffers
+from .agents.fisher import IDENTIFIED, FisherObjective
 from .rollout import RolloutEngine
 
 
@@ -41,9 +43,10 @@ class ScoreRequest:
     arrival: float
     actions: torch.Tensor          # [M, T, 12] candidate excitations
     init: torch.Tensor             # [37] or [M * (P + 1), 37]
-    theta: np.ndarray
-    explore: Sequence[str] = ("mass",)
-    delta: float = 0.1
+    theta: np.ndarray              # [25] parameters the excitation is scored at
+    explore: Optional[Sequence[str]] = None     # default: the 22 identified parameters
+    mode: str = "exact"            # "exact": logdet of the exact Fisher information;
+    delta: float = 0.1             # "fd": the reference's finite-difference FIM trace
 
 
 @dataclass
@@ -149,7 +152,7 @@ class SysIdServer:
                 batch = None
             active = keep
             for job_id, out in scoring:          # the synchronize above covers the scoring stream
-                stats.results[job_id] = {"fim": out.cpu().numpy()}
+                stats.results[job_id] = {"score": out.cpu().numpy()}
                 stats.latency_s[job_id] = time.perf_counter() - t0 - arrived[job_id]
             scoring = []
             if not active and i < len(pending):
@@ -165,8 +168,15 @@ class SysIdServer:
     def _launch_score(self, r: ScoreRequest):
         self.score_stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(self.score_stream):
-            obj = FDFIMObjective(self.engine, r.theta, list(r.explore), delta=r.delta)
-            out = obj(r.actions, r.init)
+            if r.mode == "exact":
+                obj = FisherObjective(self.engine, list(r.explore) if r.explore else IDENTIFIED)
+                th = torch.as_tensor(np.asarray(r.theta, np.float32).reshape(-1, 1), device=self.engine.device)
+                out = obj.score(th.contiguous(), r.actions, r.init).logdet
+            elif r.mode == "fd":
+                obj = FDFIMObjective(self.engine, r.theta, list(r.explore or ("mass",)), delta=r.delta)
+                out = obj(r.actions, r.init)
+            else:
+                raise ValueError(f"unknown scoring mode {r.mode!r}")
         return (r.job_id, out)
 
 

@@ -1,0 +1,281 @@
+"""Measured workloads besides the 22-parameter CEM solve (``bench.py --workload ...``).
+
+* ``landscape`` -- plain method: base-mass landscape, 256 masses on [5, 9] kg x 64 segments
+  x H 5, argmin; a latency-bound solve, replayed as one HIP graph (scripts/mass_landscape.py).
+* ``batch64`` -- 64 independent CEM problems of uneven size (horizons {5, 10, 20, 50},
+  samples {512, 2048, 8192}, segments {32, 128, 512}, fixed seed) in one persistent launch per
+  phase, launch shape from the autotuner, iterations replayed from a HIP graph.
+* ``fisher`` -- exact trajectory sensitivities of H 50 rollouts for 22 parameters x 4096
+  samples x 128 Fourier excitations (8 terms per joint, U[-0.3, 0.3]), Fisher assembly on
+  matrix cores, FP64 logdet per excitation.
+* ``serve`` -- Poisson stream of sysID and excitation-scoring jobs through the
+  continuous-batching server under an HBM budget.
+
+Each returns the dict ``bench.py`` prints as its JSON line.
+"""
+from __future__ import annotations
+
+import time
+from typing import Dict
+
+import numpy as np
+import torch
+
+from .agents.batch import BatchCEM, ProblemSpec
+from .agents.fisher import FIM_ROWS, FisherObjective, fourier_excitations
+from .agents.sysid import SysIdProblem, ground_truth, identification_prior
+from .data import standing_state, synthetic_segments
+from .params import P_MASS, PARAM_NAMES, SimParams, nominal_params
+from .model import load_model
+from .rollout import RolloutEngine
+
+PEAK_FP32_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def _events(stream):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _settled_state(eng: RolloutEngine, theta: np.ndarray, steps: int = 50) -> np.ndarray:
+    s = eng.simulate_np(np.asarray(theta)[None], standing_state(0.30)[None], np.zeros((1, steps, 12), np.float32))
+    return s[0, -1]
+
+
+# ------------------------------------------------------------------ plain method (1-D landscape)
+def landscape(steps: int, warmup: int) -> Dict:
+    sim = SimParams()
+    eng = RolloutEngine(sim)
+    model = load_model()
+    truth = nominal_params(model)                        # base-link mass 6.921 kg
+    segs = synthetic_segments(eng.simulate_np, truth, sim, n_seg=64, horizon=5, seed=0)
+    seg_dev = eng.upload_segments(segs)
+    masses = np.linspace(5.0, 9.0, 256)
+    rows = np.repeat(truth[None], 256, axis=0)
+    rows[:, P_MASS] = masses
+    theta = torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).cuda()
+    prob = SysIdProblem(eng, seg_dev, 5, max_cand=256)
+    holder = {}
+
+    def solve():
+        holder["cost"] = prob(theta, 256)
+
+    for _ in range(max(warmup, 1)):
+        solve()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        with torch.cuda.graph(g, stream=s):
+            solve()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    lat = []
+    st = torch.cuda.current_stream()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a, b = _events(st)
+        a.record(st)
+        g.replay()
+        idx_dev = torch.argmin(holder["cost"])     # eager, outside the graph
+        b.record(st)
+        lat.append((a, b))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = np.array([a.elapsed_time(b) for a, b in lat])
+    idx = int(idx_dev.item())
+    ctrl = 256 * 64 * 5
+    return {"metric": "env-steps/s, landscape solves/s; p50/p99 solve latency", "value": round(steps * ctrl / total, 1),
+            "unit": "env-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": round(total / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "landscape_mass_256x64_h5", "candidates": 256, "segments": 64, "horizon": 5,
+                       "grid_kg": [5.0, 9.0], "control_steps_per_solve": ctrl, "graph": True},
+            "solves_per_s": round(steps / total, 2),
+            "solve_latency_ms": {"p50": round(float(np.percentile(ms, 50)), 4),
+                                 "p99": round(float(np.percentile(ms, 99)), 4)},
+            "argmin_mass_kg": float(masses[idx]), "truth_kg": float(truth[P_MASS]),
+            "grid_step_kg": float(masses[1] - masses[0])}
+
+
+# ------------------------------------------------------------------ batched uneven problems
+def batch64(steps: int, warmup: int, iterations: int = 12, n_prob: int = 64, seed: int = 0) -> Dict:
+    sim = SimParams()
+    eng = RolloutEngine(sim)
+    prior = identification_prior()
+    rng = np.random.default_rng(seed)
+    Hs = rng.choice([5, 10, 20, 50], size=n_prob)
+    Ns = rng.choice([512, 2048, 8192], size=n_prob)
+    Ss = rng.choice([32, 128, 512], size=n_prob)
+    specs, truths = [], []
+    for k in range(n_prob):
+        truth = ground_truth(prior, seed=1000 + k)
+        segs = synthetic_segments(eng.simulate_np, truth, sim, n_seg=int(Ss[k]), horizon=int(Hs[k]), seed=2000 + k)
+        specs.append(ProblemSpec(eng.upload_segments(segs), int(Hs[k]), prior, n_samples=int(Ns[k]), seed=k))
+        truths.append(truth)
+    ctrl_per_iter = int(sum(int(n) * int(s) * int(h) for n, s, h in zip(Ns, Ss, Hs)))
+    tune = BatchCEM(eng, specs)
+    info = tune.autotune()
+    block, unroll = info["best"]
+    del tune
+    lat = []
+    results = None
+    st = torch.cuda.current_stream()
+    t_total = 0.0
+    for rep in range(warmup + steps):
+        b = BatchCEM(eng, specs, block=block, unroll=unroll)      # fresh problem state per solve
+        torch.cuda.synchronize()
+        a, e = _events(st)
+        t0 = time.perf_counter()
+        a.record(st)
+        b.solve(iterations, graph=True)
+        e.record(st)
+        torch.cuda.synchronize()
+        if rep >= warmup:
+            t_total += time.perf_counter() - t0
+            lat.append(a.elapsed_time(e))
+            results = b.results()
+    mass_err = [abs(r["mean"][P_MASS] - t[P_MASS]) / t[P_MASS] for r, t in zip(results, truths)]
+    ctrl = ctrl_per_iter * iterations
+    return {"metric": "env-steps/s, sysID solves/s (64 problems per batch); p50/p99 batch latency",
+            "value": round(steps * ctrl / t_total, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(t_total / steps * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "batch64_uneven", "problems": n_prob, "iterations": iterations,
+                       "horizons": [5, 10, 20, 50], "samples": [512, 2048, 8192], "segments": [32, 128, 512],
+                       "seed": seed, "control_steps_per_batch": ctrl, "substeps_per_batch": ctrl * sim.decimation,
+                       "autotune": info},
+            "solves_per_s": round(steps * n_prob / t_total, 3),
+            "batch_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                                 "p99": round(float(np.percentile(lat, 99)), 2)},
+            "base_mass_rel_err": {"median": round(float(np.median(mass_err)), 5),
+                                  "max": round(float(np.max(mass_err)), 5)}}
+
+
+# ------------------------------------------------------------------ Fisher information
+def fisher(steps: int, warmup: int, n_samples: int = 4096, n_exc: int = 128, horizon: int = 50) -> Dict:
+    sim = SimParams()
+    eng = RolloutEngine(sim)
+    prior = identification_prior()
+    rng = np.random.default_rng(0)
+    rows = np.repeat(prior.mean[None], n_samples, axis=0)
+    rows[:, prior.free] += rng.normal(size=(n_samples, int(prior.free.sum()))) * prior.std[prior.free]
+    rows = np.clip(rows, prior.lo, prior.hi)
+    theta = torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).cuda()
+    init = torch.from_numpy(_settled_state(eng, prior.mean).astype(np.float32)).cuda()
+    acts = fourier_excitations(n_exc, horizon, sim.dt * sim.decimation, sim.action_scale, n_terms=8, coeff=0.3,
+                               seed=0)
+    obj = FisherObjective(eng)
+    st = torch.cuda.current_stream()
+    chunk_exc = 16
+    sens_ms, asm_ms, ld_ms, totals = [], [], [], []
+    scores = None
+    for rep in range(warmup + steps):
+        F = torch.empty((n_exc * n_samples, obj.P, obj.P), dtype=torch.float32, device="cuda")
+        status = torch.empty(n_exc * n_samples, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev = []
+        for e0 in range(0, n_exc, chunk_exc):
+            a0, a1 = _events(st)
+            b1 = torch.cuda.Event(enable_timing=True)
+            a0.record(st)
+            J, s_, _ = obj.sensitivities(theta, acts[e0:e0 + chunk_exc], init)
+            a1.record(st)
+            obj.assemble(J, out=F[e0 * n_samples:(e0 + chunk_exc) * n_samples])
+            b1.record(st)
+            status[e0 * n_samples:(e0 + chunk_exc) * n_samples] = s_
+            ev.append((a0, a1, b1))
+            del J
+        c0, c1 = _events(st)
+        c0.record(st)
+        scores = obj.logdet(F, status, n_exc)
+        c1.record(st)
+        torch.cuda.synchronize()
+        if rep >= warmup:
+            totals.append(time.perf_counter() - t0)
+            sens_ms.append(sum(a.elapsed_time(b) for a, b, _ in ev))
+            asm_ms.append(sum(b.elapsed_time(c) for _, b, c in ev))
+            ld_ms.append(c0.elapsed_time(c1))
+    G = n_samples * n_exc
+    ctrl = G * horizon
+    t = float(np.mean(totals))
+    asm_s = float(np.mean(asm_ms)) / 1e3
+    flops_asm = 2.0 * horizon * FIM_ROWS * obj.P * obj.P * G
+    bytes_j = 4.0 * G * horizon * FIM_ROWS * obj.P
+    ld = scores.logdet.cpu().numpy()
+    valid = scores.n_valid.cpu().numpy()
+    return {"metric": "sensitivity rollouts/s (22 tangents each), excitations scored/s",
+            "value": round(ctrl / t, 1), "unit": "control steps/s (x22 tangents)", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(t * 1e3, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 (logdet fp64)", "data": "synthetic",
+            "config": {"workload": "fisher_22x4096x128_h50", "params": obj.P, "samples": n_samples,
+                       "excitations": n_exc, "horizon": horizon, "fourier_terms": 8, "coeff": 0.3,
+                       "excitation_chunk": chunk_exc},
+            "phase_ms": {"sensitivities": round(float(np.mean(sens_ms)), 2),
+                         "fim_assemble_mfma": round(float(np.mean(asm_ms)), 3),
+                         "logdet_fp64": round(float(np.mean(ld_ms)), 3)},
+            "fim_assemble_roofline": {"bound": "hbm", "achieved": round(bytes_j / asm_s / 1e9, 1),
+                                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(bytes_j / asm_s / 1e9 / HBM_PEAK_GBS, 4),
+                                      "useful_tflops": round(flops_asm / asm_s / 1e12, 2)},
+            "excitations_per_s": round(n_exc / t, 2),
+            "best_excitation": int(np.argmax(ld)), "logdet_range": [float(ld.min()), float(ld.max())],
+            "valid_fraction": round(float(valid.sum()) / G, 4)}
+
+
+# ------------------------------------------------------------------ serving
+def serve(n_requests: int = 10000, rate_hz: float = 200.0, budget_gb: float = 200.0, seed: int = 0,
+          sysid_frac: float = 0.8, n_seg: int = 4, iterations: int = 8) -> Dict:
+    from .serving import ScoreRequest, SysIdRequest, SysIdServer, poisson_arrivals
+    sim = SimParams()
+    eng = RolloutEngine(sim)
+    prior = identification_prior()
+    rng = np.random.default_rng(seed)
+    arrivals = poisson_arrivals(n_requests, rate_hz, seed)
+    # segment pools per horizon (each job draws its own truth's data from a pool of 16 truths)
+    pools = {}
+    init = torch.from_numpy(_settled_state(eng, prior.mean).astype(np.float32)).cuda()
+    reqs, kinds, work = [], [], 0
+    for k in range(n_requests):
+        if rng.random() < sysid_frac:
+            H = int(rng.integers(5, 51))
+            n = int(np.clip(np.exp(rng.uniform(np.log(512), np.log(8192))) // 256 * 256, 512, 8192))
+            key = (H, int(rng.integers(16)))
+            if key not in pools:
+                truth = ground_truth(prior, seed=5000 + key[1])
+                segs = synthetic_segments(eng.simulate_np, truth, sim, n_seg=n_seg, horizon=H, seed=6000 + key[1])
+                pools[key] = eng.upload_segments(segs)
+            reqs.append(SysIdRequest(k, float(arrivals[k]), ProblemSpec(pools[key], H, prior, n_samples=n, seed=k),
+                                     iterations=iterations))
+            kinds.append("sysid")
+            work += n * n_seg * H * iterations
+        else:
+            M = int(rng.integers(16, 129))
+            acts = fourier_excitations(M, 50, sim.dt * sim.decimation, sim.action_scale, seed=k)
+            reqs.append(ScoreRequest(k, float(arrivals[k]), acts, init, prior.mean))
+            kinds.append("score")
+            work += M * 50
+    srv = SysIdServer(eng, hbm_budget_bytes=budget_gb * 1e9)
+    stats = srv.run(reqs)
+    lat = stats.latency_s
+    by = {}
+    for kind in ("sysid", "score"):
+        v = np.array([lat[r.job_id] for r, kd in zip(reqs, kinds) if kd == kind])
+        if v.size:
+            by[kind] = {"n": int(v.size), "p50_ms": round(float(np.percentile(v, 50)) * 1e3, 2),
+                        "p99_ms": round(float(np.percentile(v, 99)) * 1e3, 2)}
+    s = stats.summary()
+    n_sysid = sum(1 for kd in kinds if kd == "sysid")
+    return {"metric": "serving: jobs/s, sysID solves/s, p50/p99 latency", "value": round(s["jobs_per_s"], 2),
+            "unit": "jobs/s", "n_gpus": 1, "steps": n_requests, "warmup": 0,
+            "ms_per_step": round(s["wall_s"] / n_requests * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "serve_poisson", "requests": n_requests, "rate_hz": rate_hz, "budget_gb": budget_gb,
+                       "sysid_fraction": sysid_frac, "sysid_samples": [512, 8192], "sysid_horizon": [5, 50],
+                       "sysid_segments": n_seg, "sysid_iterations": iterations, "score_candidates": [16, 128],
+                       "score_horizon": 50, "seed": seed},
+            "latency_ms": {"p50": round(s["p50_ms"], 2), "p99": round(s["p99_ms"], 2), "by_kind": by},
+            "sysid_solves_per_s": round(n_sysid / s["wall_s"], 2), "wall_s": round(s["wall_s"], 2),
+            "offered_work_control_steps": int(work), "ticks": s["ticks"], "max_active": s["max_active"]}

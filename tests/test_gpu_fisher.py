@@ -1,0 +1,152 @@
+"""Exact sensitivities, Fisher assembly on matrix cores, FP64 logdet and the rollout-cost
+gradient against the FP64 oracle's forward-mode derivatives, central differences and numpy."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _settled(sim, theta):
+    import spio
+    from spi_amd.data import standing_state
+    from spi_amd.model import load_model
+    o = spio.Oracle(load_model(), sim)
+    st, _ = o.simulate(theta, standing_state(0.30), np.zeros((40, 12), np.float32))
+    return o, st[-1]
+
+
+def _samples(prior, n, seed):
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(prior.mean[None], n, axis=0)
+    rows[:, prior.free] += rng.normal(size=(n, int(prior.free.sum()))) * prior.std[prior.free] * 0.5
+    return np.clip(rows, prior.lo, prior.hi)
+
+
+def test_traj_sensitivity_matches_oracle_fp64(gpu, sim, theta0):
+    """d y_t / d theta of the FP32 tangent kernel vs the FP64 forward-mode oracle, 10 control
+    steps, 22 parameters, 3 samples x 2 excitations (contact active)."""
+    from spi_amd.agents.fisher import IDENTIFIED, FisherObjective, fourier_excitations
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    o, init = _settled(sim, theta0)
+    rows = _samples(prior, 3, 1)
+    H = 10
+    acts = fourier_excitations(2, H, sim.dt * sim.decimation, sim.action_scale, seed=4)
+    obj = FisherObjective(eng)
+    theta = torch.tensor(np.ascontiguousarray(rows.T), dtype=torch.float32, device="cuda")
+    J, st, y = obj.sensitivities(theta, acts, torch.tensor(init, device="cuda"), with_y=True)
+    J = J.cpu().numpy().reshape(2, 3, H, 25, 22)
+    y = y.cpu().numpy().reshape(2, 3, H, 25)
+    assert (st.cpu().numpy() == 0).all()
+    A = acts.cpu().numpy()
+    worst = 0.0
+    for e in range(2):
+        for c in range(3):
+            yr, Jr, rc = o.traj_sens(rows[c], np.arange(22), init, A[e])
+            assert rc == 0
+            assert np.abs(y[e, c] - yr).max() < 2e-3
+            # per parameter: error relative to that parameter's sensitivity scale
+            scale = np.abs(Jr).reshape(-1, 22).max(0) + 1e-9
+            err = (np.abs(J[e, c] - Jr).reshape(-1, 22).max(0) / scale)
+            worst = max(worst, float(err.max()))
+    print("max sensitivity error relative to each column's scale:", worst)
+    assert worst < 2e-2
+
+
+def test_fim_assembly_mfma_and_logdet(gpu, sim, theta0):
+    """F_g = J_g^T J_g (FP32 matrix cores) vs float64 numpy on the kernel's own J; the FP64
+    mean / Cholesky logdet vs numpy slogdet; invalid groups are skipped."""
+    from spi_amd.agents.fisher import FisherObjective
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    obj = FisherObjective(eng, reg=1e-6)
+    rng = np.random.default_rng(0)
+    for P, K in ((22, 250), (7, 33), (32, 5)):
+        G = 10
+        Jn = rng.normal(size=(G, K, P)).astype(np.float32)
+        J = torch.tensor(Jn, device="cuda")
+        F = obj.assemble(J).cpu().numpy()
+        ref = np.einsum("gki,gkj->gij", Jn.astype(np.float64), Jn.astype(np.float64))
+        assert np.allclose(F, ref, rtol=2e-5, atol=2e-5 * np.abs(ref).max()), (P, np.abs(F - ref).max())
+        status = torch.zeros(G, dtype=torch.int32, device="cuda")
+        status[3] = 1
+        obj.P = P
+        sc = obj.logdet(torch.tensor(F, device="cuda"), status, 2)
+        for s in range(2):
+            keep = [g for g in range(5 * s, 5 * s + 5) if g != 3]
+            m = F[keep].astype(np.float64).mean(0) + 1e-6 * np.eye(P)
+            sign, ld = np.linalg.slogdet(m)
+            assert sign > 0
+            assert abs(sc.logdet[s].item() - ld) < 1e-9 * max(1.0, abs(ld)), (P, s)
+            assert sc.n_valid[s].item() == len(keep)
+            assert np.allclose(sc.fim_mean[s].cpu().numpy(), m, rtol=1e-12, atol=1e-12 * np.abs(m).max())
+
+
+def test_score_chunking_is_invariant(gpu, sim, theta0):
+    from spi_amd.agents.fisher import FisherObjective, fourier_excitations
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    _, init = _settled(sim, theta0)
+    theta = torch.tensor(np.ascontiguousarray(_samples(prior, 16, 2).T), dtype=torch.float32, device="cuda")
+    acts = fourier_excitations(4, 8, sim.dt * sim.decimation, sim.action_scale, seed=1)
+    obj = FisherObjective(eng, reg=1e-9)
+    i0 = torch.tensor(init, device="cuda")
+    a = obj.score(theta, acts, i0)
+    b = obj.score(theta, acts, i0, max_sens_bytes=16 * 8 * 25 * 22 * 4)      # one excitation per chunk
+    assert torch.equal(a.logdet, b.logdet) and torch.equal(a.fim_mean, b.fim_mean)
+    assert torch.isfinite(a.logdet).all()
+
+
+def test_cost_gradient_matches_oracle_and_central_differences(gpu, sim, theta0):
+    """Gradient of the tracking cost (22 parameters) vs the FP64 oracle gradient on 4 samples,
+    and the autograd face vs central differences (eps = 1e-4 relative) of the FP32 primal
+    kernel on 32 samples, projected on a random direction."""
+    import spio
+    from spi_amd.agents.fisher import IDENTIFIED, RolloutCost, _rows
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.model import load_model
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    segs = synthetic_segments(eng.simulate_np, ground_truth(prior, seed=2), sim, n_seg=4, horizon=5, seed=3)
+    seg_dev = eng.upload_segments(segs)
+    rows = _rows(IDENTIFIED)
+    # oracle gradient, per segment
+    o = spio.Oracle(load_model(), sim)
+    cand = _samples(prior, 4, 5)
+    theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
+    w = torch.full((4,), 0.25, device="cuda")
+    th = theta.clone().requires_grad_(True)
+    cost = RolloutCost.apply(th, eng, seg_dev, 5, w, rows)
+    cost.sum().backward()
+    g = th.grad.cpu().numpy()
+    for c in range(4):
+        cr, gr = o.cost_grad(cand[c], rows, segs.init, segs.actions, segs.targets, 0, eng.weights)
+        assert abs(cost[c].item() - 0.25 * cr.sum()) < 1e-3 * abs(0.25 * cr.sum())
+        ref = 0.25 * gr.sum(0)
+        err = np.abs(g[rows, c] - ref) / (np.abs(ref) + 1e-3 * np.abs(ref).max())
+        assert err.max() < 2e-2, (c, err.max())
+    # central differences of the primal FP32 kernel along random directions
+    cand = _samples(prior, 32, 6)
+    theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
+    th = theta.clone().requires_grad_(True)
+    RolloutCost.apply(th, eng, seg_dev, 5, w, rows).sum().backward()
+    rng = np.random.default_rng(7)
+    d = np.zeros_like(cand.T)
+    d[rows] = rng.normal(size=(22, 32)) * np.abs(cand.T[rows]).clip(1e-2, None)
+    eps = 1e-4
+    dt = torch.tensor(d, dtype=torch.float32, device="cuda")
+    fp = (w[:, None] * eng.segment_costs((theta + eps * dt).contiguous(), 32, seg_dev, 5)).sum(0)
+    fm = (w[:, None] * eng.segment_costs((theta - eps * dt).contiguous(), 32, seg_dev, 5)).sum(0)
+    fd = ((fp - fm) / (2 * eps)).cpu().numpy()
+    an = (th.grad.cpu().numpy() * d).sum(0)
+    rel = np.abs(fd - an) / (np.abs(an) + 1e-2 * np.abs(an).max())
+    print("directional derivative rel err (median, max):", np.median(rel), rel.max())
+    # FP32 central differences carry rounding noise of ~1e-6 relative / eps
+    assert np.median(rel) < 5e-2 and np.corrcoef(fd, an)[0, 1] > 0.999

@@ -19,6 +19,7 @@ def test_serving_mix_equals_alone(gpu, sim, theta0):
     import torch
     from spi_amd.agents.active import FDFIMObjective
     from spi_amd.agents.batch import BatchCEM, _Buffers
+    from spi_amd.agents.fisher import FisherObjective
     from spi_amd.agents.sysid import identification_prior
     from spi_amd.rollout import RolloutEngine
     from spi_amd.serving import ScoreRequest, SysIdRequest, SysIdServer
@@ -34,7 +35,8 @@ def test_serving_mix_equals_alone(gpu, sim, theta0):
     acts = torch.tensor(rng.uniform(-1, 1, size=(4, 6, 12)), dtype=torch.float32, device="cuda")
     init = torch.tensor(np.r_[0, 0, 0.3, 0, 0, 0, 1, np.zeros(6), sim.q0, np.zeros(12)],
                         dtype=torch.float32, device="cuda")
-    reqs.append(ScoreRequest(10, 1, acts, init, theta0, explore=("mass", "comx")))
+    reqs.append(ScoreRequest(10, 1, acts, init, theta0))
+    reqs.append(ScoreRequest(11, 2, acts, init, theta0, explore=("mass", "comx"), mode="fd"))
     # a budget that holds the two largest jobs but not three: job 2 / 3 must queue
     fp = sorted(_Buffers.footprint(s) for s, _ in specs.values())
     budget = fp[-1] + fp[-2] + fp[0] // 2
@@ -51,8 +53,11 @@ def test_serving_mix_equals_alone(gpu, sim, theta0):
         for key in ("mean", "std", "best"):
             assert np.array_equal(r[key], got[key]), (job_id, key)
         assert r["best_cost"] == got["best_cost"], job_id
+    th = torch.tensor(np.asarray(theta0, np.float32).reshape(-1, 1), device="cuda")
+    ld = FisherObjective(eng).score(th, acts, init).logdet.cpu().numpy()
+    assert np.array_equal(ld, stats.results[10]["score"])
     fim = FDFIMObjective(eng, theta0, ["mass", "comx"], delta=0.1)(acts, init).cpu().numpy()
-    assert np.array_equal(fim, stats.results[10]["fim"])
+    assert np.array_equal(fim, stats.results[11]["score"])
     s = stats.summary()
     assert s["completed"] == len(reqs) and s["p99_ms"] >= s["p50_ms"] > 0
 

@@ -34,17 +34,18 @@ def test_library_exports_every_header_symbol():
 
 def test_ctypes_structs_match_c_layout():
     """sizeof/offsetof of SpiSimArgs and SpiCostArgs as compiled by the host compiler."""
-    from spi_amd._lib import SpiCostArgs, SpiSimArgs
+    from spi_amd._lib import SpiCostArgs, SpiSimArgs, SpiTangentSpec
     from spi_amd.agents.batch import SpiBatchProblem
     src = r"""
     #include <cstdio>
     #include <cstddef>
     #include "spi_hip.h"
     int main() {
-      std::printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs), offsetof(SpiSimArgs, effort),
-                  offsetof(SpiSimArgs, motor_model), offsetof(SpiSimArgs, state_limit), sizeof(SpiCostArgs),
-                  offsetof(SpiCostArgs, w), sizeof(SpiBatchProblem), offsetof(SpiBatchProblem, seed),
-                  offsetof(SpiBatchProblem, mean), offsetof(SpiBatchProblem, best_cost));
+      std::printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs),
+                  offsetof(SpiSimArgs, effort), offsetof(SpiSimArgs, motor_model), offsetof(SpiSimArgs, state_limit),
+                  sizeof(SpiCostArgs), offsetof(SpiCostArgs, w), sizeof(SpiBatchProblem),
+                  offsetof(SpiBatchProblem, seed), offsetof(SpiBatchProblem, mean),
+                  offsetof(SpiBatchProblem, best_cost), sizeof(SpiTangentSpec), offsetof(SpiTangentSpec, param));
     }
     """
     with tempfile.TemporaryDirectory() as d:
@@ -57,7 +58,7 @@ def test_ctypes_structs_match_c_layout():
     expect = [ctypes.sizeof(SpiSimArgs), SpiSimArgs.effort.offset, SpiSimArgs.motor_model.offset,
               SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset,
               ctypes.sizeof(SpiBatchProblem), SpiBatchProblem.seed.offset, SpiBatchProblem.mean.offset,
-              SpiBatchProblem.best_cost.offset]
+              SpiBatchProblem.best_cost.offset, ctypes.sizeof(SpiTangentSpec), SpiTangentSpec.param.offset]
     assert got == expect
 
 

@@ -18,8 +18,9 @@ def segs_small(model, sim):
     return synthetic_segments(simulate, ground_truth(prior, seed=1), sim, n_seg=3, horizon=4, seed=2), prior
 
 
-@pytest.mark.parametrize("motor,cost_mode", [(3, 0), (0, 1), (1, 0), (2, 0)])
-def test_plain_matches_oracle_fp64(model, sim, segs_small, motor, cost_mode):
+@pytest.mark.parametrize("motor,cost_mode,products", [(3, 0, "bmm"), (0, 1, "bmm"), (1, 0, "bmm"), (2, 0, "bmm"),
+                                                      (3, 0, "bcast")])
+def test_plain_matches_oracle_fp64(model, sim, segs_small, motor, cost_mode, products):
     import dataclasses
     import spio
     from spi_amd.baselines.plain_torch import PlainTorchRollout
@@ -34,7 +35,8 @@ def test_plain_matches_oracle_fp64(model, sim, segs_small, motor, cost_mode):
     w = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05]) if cost_mode == 0 else np.array([10.0, 5.0, 0, 0, 1.0, 0])
     ref = spio.Oracle(model, sim).rollout_cost(rows, segs.init, segs.actions, segs.targets, cost_mode, w)
     f = lambda x: torch.tensor(np.asarray(x), dtype=torch.float64)
-    pl = PlainTorchRollout(sim, model, cost_mode=cost_mode, weights=w, device="cpu", dtype=torch.float64)
+    pl = PlainTorchRollout(sim, model, cost_mode=cost_mode, weights=w, device="cpu", dtype=torch.float64,
+                           products=products)
     got = pl.segment_costs(f(rows.T), f(segs.init), f(segs.actions), f(segs.targets)).numpy().T
     # same model in float64; the two differ only by the fixed-joint folding and summation order
     assert np.allclose(got, ref, rtol=1e-9, atol=1e-12), np.abs(got - ref).max()
