@@ -252,11 +252,14 @@ int spi_fim_assemble(const float* J_dev, long long n_groups, int n_rows, int n_t
                      hipStream_t stream);
 
 /* Per set s of set_size consecutive groups (one excitation): mean of the groups' F (FP64,
- * fixed order, groups with status != 0 skipped), plus reg * I, FP64 Cholesky, logdet.
+ * fixed order, groups with status != 0 skipped), scaled to D F D with D = diag(scale_dev)
+ * (scale_dev [n_tan] may be NULL: D = I; with D = parameter magnitudes F is the information
+ * about relative parameter changes), plus reg * trace / n_tan * I, FP64 Cholesky, logdet.
  * logdet_dev [n_sets] (-inf when not positive definite or no valid group), n_valid_dev
- * [n_sets], F_mean_dev [n_sets][n_tan][n_tan] (may be NULL). */
-int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan, double reg,
-                   double* F_mean_dev, double* logdet_dev, int* n_valid_dev, hipStream_t stream);
+ * [n_sets], F_mean_dev [n_sets][n_tan][n_tan] (the scaled, regularised mean; may be NULL). */
+int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan,
+                   const double* scale_dev, double reg, double* F_mean_dev, double* logdet_dev, int* n_valid_dev,
+                   hipStream_t stream);
 
 /* Tracking cost and its exact gradient for every (segment, candidate) pair (layouts as
  * spi_rollout_cost): cost_dev [n_seg][n_cand], grad_dev [n_seg][n_cand][n_tan] =
@@ -264,6 +267,25 @@ int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int se
 int spi_rollout_cost_grad(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
                           const float* params_dev, int ld, int n_cand, const float* segments_dev, int n_seg,
                           int horizon, float* cost_dev, float* grad_dev, hipStream_t stream);
+
+/* Weighted tracking residuals and their Jacobian, for Gauss-Newton refinement of identified
+ * parameters.  Group g = c * n_seg + s; Jr_dev [G][horizon * 37][n_tan + 1]: row t * 37 + i is
+ * [d r_i / d theta[tan->param[k]] (k < n_tan) | r_i] with r_i = sqrt(w_group(i) * seg_w[s]) *
+ * (x_i - target_i) over the 37-float state after control step t (tracking cost = sum r^2, the
+ * objective of scripts/eval.py:287-292 in squared form).  status_dev [G] as above.
+ * n_tan + 1 <= 32; tracking cost mode only. */
+int spi_residual_jacobian(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
+                          const float* params_dev, int ld, int n_cand, const float* segments_dev,
+                          const float* seg_weight_dev, int n_seg, int horizon, float* Jr_dev, int* status_dev,
+                          hipStream_t stream);
+
+/* Levenberg-Marquardt step per candidate from the augmented Gram matrices gram_dev
+ * [n_cand * n_seg][n_tan + 1][n_tan + 1] of spi_residual_jacobian's groups (spi_fim_assemble):
+ * FP64 sums over segments, (A + lambda diag(A)) delta = -J^T r by FP64 Cholesky.  delta_dev
+ * [n_cand][n_tan], cost_dev [n_cand] = r^T r (+inf if a group diverged), pred_dev [n_cand] = the
+ * linear model's cost after the step.  Replaces the Optuna TPE refinement of mass_opt.py:203-219. */
+int spi_gn_step(const float* gram_dev, const int* status_dev, int n_cand, int n_seg, int n_tan, const double* lambda_dev,
+                double* delta_dev, double* cost_dev, double* pred_dev, hipStream_t stream);
 
 /* Excitations as Fourier series: actions_dev[e][t][j] = scale * sum_k coeff_dev[e][j][k] *
  * sin(2 pi (k + 1) t dt_ctrl / period), k < n_terms. */

@@ -271,11 +271,11 @@ static void dual_params(const double* par, const int* idx, int P, DualP* p) {
   for (int m = 0; m < P; ++m) p[idx[m]].d[m] = 1.0;
 }
 
-// Trajectory and sensitivities: after control step t, y = [root (13) | q (12)] and
-// J[t][i][m] = d y[i] / d par[idx[m]].  y_out [T][25], J_out [T][25][P].
+// Trajectory and sensitivities: after control step t, y = [root (13) | q (12) | qd (12)][:n_rows]
+// and J[t][i][m] = d y[i] / d par[idx[m]].  y_out [T][n_rows], J_out [T][n_rows][P], n_rows 25 or 37.
 int spio_traj_sens(void* hp, const double* par, const int* idx, int P, const float* init37, const float* actions,
-                   int n_steps, double* y_out, double* J_out) {
-  if (P < 1 || P > P_COUNT) return -1;
+                   int n_steps, int n_rows, double* y_out, double* J_out) {
+  if (P < 1 || P > P_COUNT || n_rows < 1 || n_rows > 37) return -1;
   auto* h = static_cast<Handle<double>*>(hp);
   Engine<DualP> eng;
   dual_engine(h, eng);
@@ -288,10 +288,10 @@ int spio_traj_sens(void* hp, const double* par, const int* idx, int P, const flo
     DualP a[12];
     for (int j = 0; j < 12; ++j) a[j] = DualP(double(actions[12 * t + j]));
     eng.control_step(st, a, p);
-    for (int i = 0; i < 25; ++i) {
-      const DualP& y = i < 13 ? st.root[i] : st.q[i - 13];
-      y_out[t * 25 + i] = y.v;
-      for (int m = 0; m < P; ++m) J_out[(size_t(t) * 25 + i) * P + m] = y.d[m];
+    for (int i = 0; i < n_rows; ++i) {
+      const DualP& y = i < 13 ? st.root[i] : (i < 25 ? st.q[i - 13] : st.qd[i - 25]);
+      y_out[size_t(t) * n_rows + i] = y.v;
+      for (int m = 0; m < P; ++m) J_out[(size_t(t) * n_rows + i) * P + m] = y.d[m];
     }
   }
   return finite_state(st) ? 0 : 1;

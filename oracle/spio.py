@@ -74,7 +74,7 @@ def lib():
         L.spio_simulate.argtypes = [vp, dp, fp, fp, C.c_int, fp]
         L.spio_rollout_cost.argtypes = [vp, dp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp]
         ipp = C.POINTER(C.c_int)
-        L.spio_traj_sens.argtypes = [vp, dp, ipp, C.c_int, fp, fp, C.c_int, dp, dp]
+        L.spio_traj_sens.argtypes = [vp, dp, ipp, C.c_int, fp, fp, C.c_int, C.c_int, dp, dp]
         L.spio_cost_grad.argtypes = [vp, dp, ipp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp, dp]
         L.spio_rollout_cost_f32.argtypes = [vp, fp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, fp,
                                             C.c_int]
@@ -280,17 +280,17 @@ class Oracle:
                                 n_seg, H, cost_mode, _dp(w), _dp(out))
         return out
 
-    def traj_sens(self, par, idx, init37, actions):
-        """FP64 forward-mode sensitivities: y [T, 25] (root 13 + q 12 after each control step)
-        and J [T, 25, P] = d y / d par[idx]."""
+    def traj_sens(self, par, idx, init37, actions, n_rows=25):
+        """FP64 forward-mode sensitivities: y [T, n_rows] (root 13 + q 12 (+ qd 12) after each
+        control step) and J [T, n_rows, P] = d y / d par[idx]."""
         par = np.ascontiguousarray(par, dtype=np.float64)
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         init37 = np.ascontiguousarray(init37, dtype=np.float32)
         actions = np.ascontiguousarray(actions, dtype=np.float32)
         T, P = actions.shape[0], idx.shape[0]
-        y, J = np.zeros((T, 25)), np.zeros((T, 25, P))
+        y, J = np.zeros((T, n_rows)), np.zeros((T, n_rows, P))
         rc = lib().spio_traj_sens(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), P, _fp(init37),
-                                  _fp(actions), T, _dp(y), _dp(J))
+                                  _fp(actions), T, n_rows, _dp(y), _dp(J))
         return y, J, rc
 
     def cost_grad(self, par, idx, seg_init, seg_actions, seg_targets, cost_mode, weights):
@@ -342,3 +342,50 @@ def cem_sample(mean, std, lo, hi, n, seed, iteration):
     out = np.zeros((mean.shape[0], n))
     lib().spio_cem_sample(_dp(mean), _dp(std), _dp(lo), _dp(hi), mean.shape[0], n, seed, iteration, _dp(out))
     return out
+
+
+def gauss_newton_fp64(o: "Oracle", theta0, rows, segs, weights, seg_w, lo=None, hi=None, iterations=8, lam0=1e-3,
+                      tol=1e-10):
+    """FP64 Levenberg-Marquardt on the tracking residuals with the oracle's exact derivatives;
+    the same damping / acceptance rule as the GPU refiner (spi_amd.agents.refine)."""
+    rows = np.asarray(rows, dtype=np.int32)
+    w = np.asarray(weights, np.float64)
+    grp = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
+
+    def resid(theta, with_jac):
+        R, Js = [], []
+        for s in range(segs.n_seg):
+            y, J, rc = o.traj_sens(theta, rows, segs.init[s], segs.actions[s], n_rows=37)
+            sw = np.sqrt(w[grp] * seg_w[s])
+            R.append((sw * (y - segs.targets[s].astype(np.float64))).ravel())
+            if with_jac:
+                Js.append((sw[None, :, None] * J).reshape(-1, len(rows)))
+        return np.concatenate(R), (np.concatenate(Js) if with_jac else None)
+
+    theta = np.asarray(theta0, np.float64).copy()
+    r, _ = resid(theta, False)
+    cost = float(r @ r)
+    lam = lam0
+    history = [cost]
+    for _ in range(iterations):
+        r, J = resid(theta, True)
+        A, b = J.T @ J, J.T @ r
+        eps = 1e-12 * np.trace(A) / len(rows)
+        Ad = A + lam * np.diag(np.diag(A)) + eps * np.eye(len(rows))
+        delta = -np.linalg.solve(Ad, b)
+        cand = theta.copy()
+        cand[rows] += delta
+        if lo is not None:
+            cand = np.clip(cand, lo, hi)
+        rn, _ = resid(cand, False)
+        c_new = float(rn @ rn)
+        if np.isfinite(c_new) and c_new < cost:
+            gain = cost - c_new
+            theta, cost = cand, c_new
+            lam = max(lam / 3.0, 1e-9)
+            history.append(cost)
+            if gain <= tol * max(cost, 1e-30):
+                break
+        else:
+            lam = min(lam * 4.0, 1e6)
+    return theta, cost, history

@@ -66,7 +66,8 @@ __global__ __launch_bounds__(256) void fim_assemble_kernel(const float* __restri
 
 // One workgroup per set: FP64 mean of the valid groups' F, + reg I, Cholesky, logdet.
 __global__ __launch_bounds__(256) void fim_logdet_kernel(const float* __restrict__ F, const int* __restrict__ status,
-                                                         int set_size, int P, double reg, double* __restrict__ Fmean,
+                                                         int set_size, int P, const double* __restrict__ scale,
+                                                         double reg, double* __restrict__ Fmean,
                                                          double* __restrict__ logdet, int* __restrict__ n_valid) {
   __shared__ double M[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
   __shared__ int cnt;
@@ -85,8 +86,15 @@ __global__ __launch_bounds__(256) void fim_logdet_kernel(const float* __restrict
       if (status[g0 + c] == 0) acc += static_cast<double>(F[(g0 + c) * P * P + e]);
     const int i = e / P, j = e % P;
     double m = n > 0 ? acc / n : 0.0;
-    if (i == j) m += reg;
+    if (scale) m *= scale[i] * scale[j];
     M[e] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {            // relative ridge: reg * mean eigenvalue
+    double tr = 0.0;
+    for (int i = 0; i < P; ++i) tr += M[i * P + i];
+    const double add = reg * (tr > 0.0 ? tr / P : 0.0);
+    for (int i = 0; i < P; ++i) M[i * P + i] += add;
   }
   __syncthreads();
   if (Fmean)
@@ -127,6 +135,75 @@ __global__ void fourier_kernel(const float* __restrict__ coeff, int n_exc, int n
   out[i] = static_cast<float>(scale * acc);
 }
 
+// Gauss-Newton / Levenberg-Marquardt step per candidate from the augmented Gram matrices
+// [J r]^T [J r] of its n_seg segment groups: FP64 sums A = J^T J, b = J^T r, rr = r^T r, then
+// (A + lambda diag(A) + 1e-12 tr(A)/P I) delta = -b by FP64 Cholesky; pred = rr - model(delta).
+__global__ __launch_bounds__(256) void gn_step_kernel(const float* __restrict__ Gm, const int* __restrict__ status,
+                                                      int n_seg, int P, const double* __restrict__ lambda,
+                                                      double* __restrict__ delta, double* __restrict__ cost,
+                                                      double* __restrict__ pred) {
+  __shared__ double M[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
+  __shared__ double Ls[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
+  __shared__ int bad;
+  const int c = blockIdx.x, L = P + 1;
+  const size_t g0 = static_cast<size_t>(c) * n_seg;
+  if (threadIdx.x == 0) {
+    int b = 0;
+    for (int s = 0; s < n_seg; ++s) b |= status[g0 + s];
+    bad = b;
+  }
+  for (int e = threadIdx.x; e < L * L; e += blockDim.x) {
+    double acc = 0.0;
+    for (int s = 0; s < n_seg; ++s) acc += static_cast<double>(Gm[(g0 + s) * L * L + e]);
+    M[e] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double rr = M[P * L + P];
+  cost[c] = bad ? INFINITY : rr;
+  double tr = 0.0;
+  for (int i = 0; i < P; ++i) tr += M[i * L + i];
+  const double lam = lambda[c], eps = 1e-12 * tr / P;
+  bool pd = !bad;
+  for (int j = 0; j < P && pd; ++j) {          // Cholesky of the damped normal matrix (lower)
+    double d = M[j * L + j] * (1.0 + lam) + eps;
+    for (int k = 0; k < j; ++k) d -= Ls[j * P + k] * Ls[j * P + k];
+    if (!(d > 0.0)) { pd = false; break; }
+    const double ljj = sqrt(d);
+    Ls[j * P + j] = ljj;
+    for (int i = j + 1; i < P; ++i) {
+      double v = 0.5 * (M[i * L + j] + M[j * L + i]);
+      for (int k = 0; k < j; ++k) v -= Ls[i * P + k] * Ls[j * P + k];
+      Ls[i * P + j] = v / ljj;
+    }
+  }
+  double* dc = delta + static_cast<size_t>(c) * P;
+  if (!pd) {
+    for (int i = 0; i < P; ++i) dc[i] = 0.0;
+    pred[c] = bad ? INFINITY : rr;
+    return;
+  }
+  double y[SPI_MAX_TANGENTS];
+  for (int i = 0; i < P; ++i) {                // L y = -b
+    double v = -0.5 * (M[i * L + P] + M[P * L + i]);
+    for (int k = 0; k < i; ++k) v -= Ls[i * P + k] * y[k];
+    y[i] = v / Ls[i * P + i];
+  }
+  for (int i = P - 1; i >= 0; --i) {           // L^T x = y
+    double v = y[i];
+    for (int k = i + 1; k < P; ++k) v -= Ls[k * P + i] * dc[k];
+    dc[i] = v / Ls[i * P + i];
+  }
+  // predicted cost of the linear model: rr + 2 b.delta + delta^T A delta
+  double q = rr;
+  for (int i = 0; i < P; ++i) {
+    double Ad = 0.0;
+    for (int k = 0; k < P; ++k) Ad += M[i * L + k] * dc[k];
+    q += 2.0 * M[i * L + P] * dc[i] + dc[i] * Ad;
+  }
+  pred[c] = q;
+}
+
 }  // namespace
 
 extern "C" {
@@ -142,14 +219,15 @@ int spi_fim_assemble(const float* J_dev, long long n_groups, int n_rows, int n_t
   return flaunch("fim_assemble_kernel");
 }
 
-int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan, double reg,
-                   double* F_mean_dev, double* logdet_dev, int* n_valid_dev, hipStream_t stream) {
+int spi_fim_logdet(const float* F_dev, const int* status_dev, int n_sets, int set_size, int n_tan,
+                   const double* scale_dev, double reg, double* F_mean_dev, double* logdet_dev, int* n_valid_dev,
+                   hipStream_t stream) {
   if (n_sets < 0 || set_size < 1 || n_tan < 1 || n_tan > SPI_MAX_TANGENTS || !(reg >= 0.0) || !F_dev ||
       !status_dev || !logdet_dev || !n_valid_dev)
     return ffail(SPI_ERR_INVALID_ARG, "spi_fim_logdet: bad arguments");
   if (n_sets == 0) return SPI_OK;
-  fim_logdet_kernel<<<n_sets, 256, 0, stream>>>(F_dev, status_dev, set_size, n_tan, reg, F_mean_dev, logdet_dev,
-                                                n_valid_dev);
+  fim_logdet_kernel<<<n_sets, 256, 0, stream>>>(F_dev, status_dev, set_size, n_tan, scale_dev, reg, F_mean_dev,
+                                                logdet_dev, n_valid_dev);
   return flaunch("fim_logdet_kernel");
 }
 
@@ -164,6 +242,17 @@ int spi_fourier_excitations(const float* coeff_dev, int n_exc, int n_terms, int 
   fourier_kernel<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(coeff_dev, n_exc, n_terms, horizon, dt_ctrl,
                                                                     period, scale, actions_dev);
   return flaunch("fourier_kernel");
+}
+
+int spi_gn_step(const float* gram_dev, const int* status_dev, int n_cand, int n_seg, int n_tan, const double* lambda_dev,
+                double* delta_dev, double* cost_dev, double* pred_dev, hipStream_t stream) {
+  if (n_cand < 0 || n_seg < 1 || n_tan < 1 || n_tan + 1 > SPI_MAX_TANGENTS || !gram_dev || !status_dev ||
+      !lambda_dev || !delta_dev || !cost_dev || !pred_dev)
+    return ffail(SPI_ERR_INVALID_ARG, "spi_gn_step: bad arguments");
+  if (n_cand == 0) return SPI_OK;
+  gn_step_kernel<<<n_cand, 256, 0, stream>>>(gram_dev, status_dev, n_seg, n_tan, lambda_dev, delta_dev, cost_dev,
+                                             pred_dev);
+  return flaunch("gn_step_kernel");
 }
 
 }  // extern "C"

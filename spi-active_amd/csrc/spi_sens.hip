@@ -183,6 +183,66 @@ __global__ __launch_bounds__(kSensThreads) void cost_grad_kernel(SpiSimArgs a, S
   if (k == 0) reinterpret_cast<unsigned int*>(out)[o] = ok ? __float_as_uint(acc.v) : 0x7f800000u;
 }
 
+// Weighted tracking residuals and their Jacobian for Gauss-Newton refinement.  Group
+// g = c * n_seg + s (candidate c, segment s) has n_tan + 1 lanes: lane k < n_tan carries the
+// tangent of parameter row tan.param[k], lane n_tan carries no tangent and writes the residual
+// itself, so row t * 37 + i of the group's matrix is [d r / d theta | r] with
+// r_i = sqrt(w_group(i) * seg_w) * (x_i - target_i) over the 37-float state after step t
+// (x = [pos, quat xyzw, linvel, angvel (world), q, qd]; the tracking cost is sum r^2).
+__global__ __launch_bounds__(kSensThreads) void residual_sens_kernel(SpiSimArgs a, SpiCostArgs cost,
+                                                                     SpiTangentSpec tan, const float* __restrict__ par,
+                                                                     int ld, int n_cand,
+                                                                     const float* __restrict__ seg,
+                                                                     const float* __restrict__ seg_w, int n_seg, int H,
+                                                                     int stride, int gpb, float* __restrict__ Jr,
+                                                                     int* __restrict__ status) {
+  const int P = tan.n_tan, L = P + 1;
+  const int gl = threadIdx.x / L, k = threadIdx.x % L;
+  const long long g = static_cast<long long>(blockIdx.x) * gpb + gl;
+  if (gl >= gpb || g >= static_cast<long long>(n_cand) * n_seg) return;
+  const int c = static_cast<int>(g / n_seg), s = static_cast<int>(g % n_seg);
+  const float* rec0 = seg + static_cast<size_t>(s) * stride;
+  const LaneParams lp = load_params_tan(par, ld, c, k < P ? tan.param[k] : -1);
+  State st;
+  init_state(st, rec0);
+  float sw[6];
+  const float ws = seg_w[s];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) sw[q] = sqrtf(fmaxf(cost.w[q] * ws, 0.0f));
+  float* Jg = Jr + static_cast<size_t>(g) * H * SPI_STATE_DIM * L + k;
+  bool ok = true;
+  for (int t = 0; t < H; ++t) {
+    const float* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+    float* row = Jg + static_cast<size_t>(t) * SPI_STATE_DIM * L;
+    if (ok) {
+      DualT act[12];
+#pragma unroll
+      for (int j = 0; j < 12; ++j) act[j] = DualT(rec[j]);
+      for (int d = 0; d < a.decimation; ++d) {
+        DualT tau[12];
+        control_torques(a, lp, act, st, tau);
+        physics_step(a, lp, st, tau, nullptr);
+      }
+      ok = state_ok(st, a.state_limit);
+    }
+    if (ok) {
+      DualT r[13], q[12], qd[12];
+      state_to_root(st, r, q, qd, 1);
+      const float* tg = rec + 12;
+#pragma unroll
+      for (int i = 0; i < SPI_STATE_DIM; ++i) {
+        const DualT& x = i < 13 ? r[i] : (i < 25 ? q[i - 13] : qd[i - 25]);
+        const int grp = i < 3 ? 0 : (i < 7 ? 1 : (i < 10 ? 2 : (i < 13 ? 3 : (i < 25 ? 4 : 5))));
+        row[i * L] = sw[grp] * (k < P ? x.d[0] : x.v - tg[i]);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < SPI_STATE_DIM; ++i) row[i * L] = 0.0f;
+    }
+  }
+  if (k == 0) status[g] = ok ? 0 : 1;
+}
+
 bool tan_ok(const SpiTangentSpec* t) {
   if (!t || t->n_tan < 1 || t->n_tan > SPI_MAX_TANGENTS || t->n_tan > SPI_P_COUNT) return false;
   for (int k = 0; k < t->n_tan; ++k)
@@ -236,6 +296,30 @@ int spi_rollout_cost_grad(const SpiSimArgs* sim, const SpiCostArgs* cost, const 
   cost_grad_kernel<<<static_cast<unsigned>(blocks), kSensThreads, lds, stream>>>(
       *sim, *cost, *tan, params_dev, ld, n_cand, segments_dev, horizon, stride, gpb, bps, cost_dev, grad_dev);
   return slaunch("cost_grad_kernel");
+}
+
+int spi_residual_jacobian(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
+                          const float* params_dev, int ld, int n_cand, const float* segments_dev,
+                          const float* seg_weight_dev, int n_seg, int horizon, float* Jr_dev, int* status_dev,
+                          hipStream_t stream) {
+  if (!sim_ok(sim)) return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian: invalid SpiSimArgs");
+  if (!tan_ok(tan) || tan->n_tan + 1 > SPI_MAX_TANGENTS)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian: bad tangent spec (n_tan + 1 <= 32)");
+  if (!cost || cost->mode != SPI_COST_TRACKING)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian: needs the tracking (sum of squares) cost");
+  if (n_cand < 0 || ld < n_cand || n_seg < 0 || horizon < 1 || !params_dev || !segments_dev || !seg_weight_dev ||
+      !Jr_dev || !status_dev)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian: bad sizes or null pointer");
+  const long long G = static_cast<long long>(n_cand) * n_seg;
+  if (G == 0) return SPI_OK;
+  const int stride = SPI_STATE_DIM + horizon * (12 + SPI_STATE_DIM);
+  const int gpb = kSensThreads / (tan->n_tan + 1);
+  const long long blocks = (G + gpb - 1) / gpb;
+  if (blocks > 0x7fffffffLL) return sfail(SPI_ERR_TOO_LARGE, "spi_residual_jacobian: grid too large");
+  residual_sens_kernel<<<static_cast<unsigned>(blocks), kSensThreads, 0, stream>>>(
+      *sim, *cost, *tan, params_dev, ld, n_cand, segments_dev, seg_weight_dev, n_seg, horizon, stride, gpb, Jr_dev,
+      status_dev);
+  return slaunch("residual_sens_kernel");
 }
 
 }  // extern "C"

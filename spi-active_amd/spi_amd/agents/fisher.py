@@ -62,12 +62,20 @@ class FisherScores:
 class FisherObjective:
     """D-optimal excitation scoring from exact sensitivities."""
 
-    def __init__(self, engine: RolloutEngine, explore: Sequence[str] = IDENTIFIED, reg: float = 1e-9):
+    def __init__(self, engine: RolloutEngine, explore: Sequence[str] = IDENTIFIED, reg: float = 1e-6,
+                 scale: Optional[Sequence[float]] = None):
+        """scale: per-parameter magnitudes (e.g. the prior mean): the logdet is then that of the
+        information about relative parameter changes, comparable across units; None = raw.
+        reg: ridge relative to the mean eigenvalue (trace / P)."""
         self.engine = engine
         self.names = list(explore)
         self.spec = _lib.tangent_spec(_rows(self.names))
         self.P = len(self.names)
         self.reg = float(reg)
+        self.scale = None if scale is None else torch.as_tensor(
+            np.abs(np.asarray(scale, np.float64)), dtype=torch.float64, device=engine.device).contiguous()
+        if self.scale is not None and self.scale.numel() != self.P:
+            raise ValueError("one scale per explored parameter")
 
     # -- the three device stages
     def sensitivities(self, theta: torch.Tensor, actions: torch.Tensor, init: torch.Tensor, with_y: bool = False):
@@ -100,7 +108,8 @@ class FisherObjective:
         ld = torch.empty(n_sets, dtype=torch.float64, device=dev)
         Fm = torch.empty((n_sets, P, P), dtype=torch.float64, device=dev)
         nv = torch.empty(n_sets, dtype=torch.int32, device=dev)
-        rc = _lib.lib().spi_fim_logdet(_lib.ptr(F), _lib.ptr(status, torch.int32), n_sets, G // n_sets, P, self.reg,
+        rc = _lib.lib().spi_fim_logdet(_lib.ptr(F), _lib.ptr(status, torch.int32), n_sets, G // n_sets, P,
+                                       _lib.ptr(self.scale, torch.float64), self.reg,
                                        _lib.ptr(Fm, torch.float64), _lib.ptr(ld, torch.float64),
                                        _lib.ptr(nv, torch.int32), _lib.stream())
         _lib.check(rc, "spi_fim_logdet")

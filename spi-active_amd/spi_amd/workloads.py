@@ -166,7 +166,7 @@ def fisher(steps: int, warmup: int, n_samples: int = 4096, n_exc: int = 128, hor
     init = torch.from_numpy(_settled_state(eng, prior.mean).astype(np.float32)).cuda()
     acts = fourier_excitations(n_exc, horizon, sim.dt * sim.decimation, sim.action_scale, n_terms=8, coeff=0.3,
                                seed=0)
-    obj = FisherObjective(eng)
+    obj = FisherObjective(eng, reg=1e-6, scale=prior.mean[:22])
     st = torch.cuda.current_stream()
     chunk_exc = 16
     sens_ms, asm_ms, ld_ms, totals = [], [], [], []

@@ -77,12 +77,21 @@ def test_fim_assembly_mfma_and_logdet(gpu, sim, theta0):
         sc = obj.logdet(torch.tensor(F, device="cuda"), status, 2)
         for s in range(2):
             keep = [g for g in range(5 * s, 5 * s + 5) if g != 3]
-            m = F[keep].astype(np.float64).mean(0) + 1e-6 * np.eye(P)
+            m = F[keep].astype(np.float64).mean(0)
+            m = m + 1e-6 * np.trace(m) / P * np.eye(P)
             sign, ld = np.linalg.slogdet(m)
             assert sign > 0
             assert abs(sc.logdet[s].item() - ld) < 1e-9 * max(1.0, abs(ld)), (P, s)
             assert sc.n_valid[s].item() == len(keep)
             assert np.allclose(sc.fim_mean[s].cpu().numpy(), m, rtol=1e-12, atol=1e-12 * np.abs(m).max())
+        # scaled: D F D
+        d = np.linspace(0.5, 2.0, P)
+        obj.scale = torch.tensor(d, dtype=torch.float64, device="cuda")
+        sc2 = obj.logdet(torch.tensor(F, device="cuda"), status, 2)
+        m = F[[0, 1, 2, 4]].astype(np.float64).mean(0) * np.outer(d, d)
+        m = m + 1e-6 * np.trace(m) / P * np.eye(P)
+        assert abs(sc2.logdet[0].item() - np.linalg.slogdet(m)[1]) < 1e-9 * max(1.0, abs(np.linalg.slogdet(m)[1]))
+        obj.scale = None
 
 
 def test_score_chunking_is_invariant(gpu, sim, theta0):

@@ -1,0 +1,85 @@
+"""Gauss-Newton refinement: residual Jacobian kernel consistent with the primal cost and the
+gradient kernel; recovers the ground truth from noise-free data; lands on the FP64 oracle's
+Levenberg-Marquardt minimiser on noisy data."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(eng, prior, sigma, n_seg, H, seed):
+    from spi_amd.agents.sysid import ground_truth
+    from spi_amd.data import synthetic_segments
+    truth = ground_truth(prior, seed=seed)
+    segs = synthetic_segments(eng.simulate_np, truth, eng.sim, n_seg=n_seg, horizon=H, seed=seed + 1,
+                              sigma_q=sigma, sigma_v=5 * sigma)
+    return truth, segs
+
+
+def test_residual_gram_matches_cost_and_gradient(gpu, sim, theta0):
+    from spi_amd.agents.fisher import IDENTIFIED, RolloutCost, _rows
+    from spi_amd.agents.refine import GaussNewtonRefiner
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    truth, segs = _data(eng, prior, 0.01, 6, 5, 11)
+    seg_dev = eng.upload_segments(segs)
+    ref = GaussNewtonRefiner(eng, seg_dev, 5)
+    theta = prior.mean.copy()
+    _, c_gn, _ = ref.step(theta, 1e-3)
+    c_primal = ref.objective(theta)
+    assert abs(c_gn - c_primal) < 1e-4 * c_primal, (c_gn, c_primal)
+    # J^T r from the augmented Gram matrix == half the gradient of the gradient kernel
+    G = ref.gram.double().sum(0).cpu().numpy()
+    P = ref.P
+    jtr = G[:P, P]
+    th = torch.tensor(theta.astype(np.float32).reshape(-1, 1), device="cuda", requires_grad=True)
+    RolloutCost.apply(th, eng, seg_dev, 5, ref.seg_w, _rows(IDENTIFIED)).sum().backward()
+    grad = th.grad.cpu().numpy()[_rows(IDENTIFIED), 0]
+    assert np.allclose(2 * jtr, grad, rtol=2e-3, atol=2e-3 * np.abs(grad).max()), np.abs(2 * jtr - grad).max()
+
+
+def test_refine_recovers_truth_without_noise(gpu, sim, theta0):
+    from spi_amd.agents.refine import GaussNewtonRefiner
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.params import P_MASS
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    truth, segs = _data(eng, prior, 0.0, 16, 10, 21)
+    ref = GaussNewtonRefiner(eng, eng.upload_segments(segs), 10, lo=prior.lo, hi=prior.hi)
+    rng = np.random.default_rng(3)
+    start = truth.copy()
+    start[:22] *= 1 + 0.03 * rng.normal(size=22)          # a CEM-sized offset
+    res = ref.refine(start, iterations=15)
+    print("cost history:", res.history)
+    assert res.cost < 1e-3 * res.history[0]
+    assert abs(res.theta[P_MASS] - truth[P_MASS]) / truth[P_MASS] < 2e-3
+
+
+def test_refine_matches_fp64_oracle_minimiser(gpu, sim, theta0):
+    import spio
+    from spi_amd.agents.fisher import IDENTIFIED, _rows
+    from spi_amd.agents.refine import GaussNewtonRefiner
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.model import load_model
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    truth, segs = _data(eng, prior, 0.01, 6, 5, 31)
+    ref = GaussNewtonRefiner(eng, eng.upload_segments(segs), 5, lo=prior.lo, hi=prior.hi)
+    rng = np.random.default_rng(4)
+    start = truth.copy()
+    start[:22] *= 1 + 0.02 * rng.normal(size=22)
+    gpu_res = ref.refine(start, iterations=6)
+    o = spio.Oracle(load_model(), sim)
+    th64, c64, hist = spio.gauss_newton_fp64(o, start, _rows(IDENTIFIED), segs, eng.weights,
+                                             np.full(6, 1 / 6), prior.lo, prior.hi, iterations=6)
+    print("gpu cost", gpu_res.history, "fp64 cost", hist)
+    assert abs(gpu_res.cost - c64) < 2e-3 * c64
+    big = np.abs(th64[:22]) > 1e-2
+    rel = np.abs(gpu_res.theta[:22] - th64[:22]) / np.abs(th64[:22])
+    print("max rel param difference (|theta| > 1e-2):", rel[big].max())
+    assert rel[big].max() < 1e-2

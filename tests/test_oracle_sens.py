@@ -53,3 +53,26 @@ def test_cost_gradient_matches_central_differences(model, sim, theta0):
         fd = (o.rollout_cost(tp[None], segs.init, segs.actions, segs.targets, 0, w)[0] -
               o.rollout_cost(tm[None], segs.init, segs.actions, segs.targets, 0, w)[0]) / (2 * h)
         assert np.allclose(fd, g[:, m], rtol=1e-5, atol=1e-6 * np.abs(g).max()), (m, fd, g[:, m])
+
+
+def test_fp64_gauss_newton_recovers_truth_without_noise(model, sim, theta0):
+    """The oracle's FP64 Levenberg-Marquardt (the reference minimiser for the GPU refiner)
+    converges to the ground truth on noise-free data, where the minimum is exactly the truth."""
+    import spio
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.params import P_MASS
+    o, _ = _setup(model, sim, theta0)
+
+    def simulate(theta, init, acts):
+        return np.stack([o.simulate(np.asarray(theta)[i], init[i], acts[i])[0] for i in range(init.shape[0])])
+    prior = identification_prior(theta0)
+    truth = ground_truth(prior, seed=8)
+    segs = synthetic_segments(simulate, truth, sim, n_seg=4, horizon=6, seed=9, sigma_q=0.0, sigma_v=0.0)
+    start = truth.copy()
+    start[:22] *= 1 + 0.02 * np.random.default_rng(1).normal(size=22)
+    w = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
+    th, cost, hist = spio.gauss_newton_fp64(o, start, np.arange(22), segs, w, np.full(4, 0.25), prior.lo, prior.hi,
+                                            iterations=12)
+    assert cost < 1e-4 * hist[0], hist
+    assert abs(th[P_MASS] - truth[P_MASS]) / truth[P_MASS] < 1e-3
