@@ -106,6 +106,26 @@ typedef struct SpiBatchProblem {
   float* best_cost;             /* [1] */
 } SpiBatchProblem;
 
+/* Command sequences of the active-exploration optimizer (active_sysid.py:259-400). */
+#define SPI_MAX_COMMAND_DIMS 16
+typedef enum SpiCommandMode {
+  SPI_COMMAND_CONSTANT = 0,     /* _sample_constant,   active_sysid.py:295-314 */
+  SPI_COMMAND_POLYNOMIAL = 1,   /* _sample_polynomial, active_sysid.py:316-361 */
+  SPI_COMMAND_BEZIER = 2        /* _sample_bezier,     active_sysid.py:363-400 */
+} SpiCommandMode;
+typedef struct SpiCommandSpec {
+  int mode;                     /* SpiCommandMode */
+  int n_dims;                   /* command width (14), <= SPI_MAX_COMMAND_DIMS */
+  int n_sampled;                /* sampled dimensions */
+  int sampled_idx[SPI_MAX_COMMAND_DIMS];
+  float default_cmd[SPI_MAX_COMMAND_DIMS];
+  float lo[SPI_MAX_COMMAND_DIMS], hi[SPI_MAX_COMMAND_DIMS];  /* command_ranges per dimension */
+  int n_updates;                /* rollout_length // horizon_length */
+  int steps_per_update;         /* int(horizon_length / dt) */
+  int poly_degree;
+  int n_bezier;
+} SpiCommandSpec;
+
 /* Tangent directions of a sensitivity rollout: slot k seeds parameter row param[k] of the
  * SoA parameter array (0 <= param[k] < SPI_P_COUNT) with a unit tangent. */
 typedef struct SpiTangentSpec {
@@ -286,6 +306,23 @@ int spi_residual_jacobian(const SpiSimArgs* sim, const SpiCostArgs* cost, const 
  * linear model's cost after the step.  Replaces the Optuna TPE refinement of mass_opt.py:203-219. */
 int spi_gn_step(const float* gram_dev, const int* status_dev, int n_cand, int n_seg, int n_tan, const double* lambda_dev,
                 double* delta_dev, double* cost_dev, double* pred_dev, hipStream_t stream);
+
+/* Command sequences of n_trials trials from their parameters, params_dev [n_trials][n_params]
+ * in the order the reference suggests them: constant [n_sampled][n_updates], polynomial
+ * [n_updates][n_sampled][poly_degree + 1] (coefficients, tanh-squashed and mapped to the
+ * range), bezier [n_updates][n_sampled][n_bezier] (control points, Bernstein basis, clipped).
+ * out_dev [n_trials][n_updates * steps_per_update][n_dims]; unsampled dimensions hold
+ * default_cmd.  Replaces the per-trial Python loops of active_sysid.py:295-400. */
+int spi_command_sequences(const SpiCommandSpec* spec, const float* params_dev, int n_trials, int n_params,
+                          float* out_dev, hipStream_t stream);
+
+/* Sliding windows of one recorded trajectory packed as segment records (layout of
+ * spi_rollout_cost): window w starts at t = first + w; init = states[t], then per step k
+ * actions[t + k] and target states[t + k + 1].  states_dev [n_steps][37], actions_dev
+ * [n_steps][12]; first + n_windows + horizon <= n_steps.  The device side of
+ * scripts/eval.py:101-171 (one window per start index, stride 1). */
+int spi_pack_windows(const float* states_dev, const float* actions_dev, int n_steps, int horizon, int first,
+                     int n_windows, float* segments_dev, hipStream_t stream);
 
 /* Excitations as Fourier series: actions_dev[e][t][j] = scale * sum_k coeff_dev[e][j][k] *
  * sin(2 pi (k + 1) t dt_ctrl / period), k < n_terms. */

@@ -73,7 +73,15 @@ class ServiceStats:
 
 
 class SysIdServer:
-    def __init__(self, engine: RolloutEngine, hbm_budget_bytes: float = 200e9, max_active: int = 256,
+    """Continuous batching on one GPU.
+
+    The host runs one tick ahead of the device: it launches CEM iteration k + 1 of every
+    admitted job while iteration k still runs, and it never synchronises the device as a whole.
+    A job that reaches its iteration count leaves the batch at once (its results are copied to
+    pinned host memory behind an event), which frees its budget for waiting jobs; scoring jobs
+    run on their own stream.  Completed copies are polled every tick."""
+
+    def __init__(self, engine: RolloutEngine, hbm_budget_bytes: float = 200e9, max_active: int = 4096,
                  block: int = 256, unroll: int = 1):
         self.engine = engine
         self.budget = float(hbm_budget_bytes)
@@ -81,13 +89,19 @@ class SysIdServer:
         self.block, self.unroll = block, unroll
         self.score_stream = torch.cuda.Stream()   # default priority, like every stream here
 
+    @staticmethod
+    def _to_host(t: torch.Tensor) -> torch.Tensor:
+        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        return h
+
     def run(self, requests: Sequence, clock: str = "wall") -> ServiceStats:
         """Serve requests until all are done.
 
         clock="wall": ``arrival`` is in seconds since the call; clock="ticks": ``arrival`` counts
         CEM iterations of the service (deterministic membership, for tests and replays).  Latency
         runs from the moment a request has arrived (queueing for memory included) to the moment
-        its result is on the host."""
+        the host sees its result in host memory."""
         if clock not in ("wall", "ticks"):
             raise ValueError(f"clock must be 'wall' or 'ticks', not {clock!r}")
         realtime = clock == "wall"
@@ -96,14 +110,33 @@ class SysIdServer:
             if isinstance(r, SysIdRequest) and _Buffers.footprint(r.spec) > self.budget:
                 raise ValueError(f"job {r.job_id} needs {_Buffers.footprint(r.spec)} B, over the whole budget")
         stats = ServiceStats()
+        main = torch.cuda.current_stream()
+        self.score_stream.wait_stream(main)       # inputs of every request are ready
         active: List[_Active] = []
-        scoring: List = []
+        inflight: List = []            # (job_id, event, host tensors dict)
         arrived: Dict[int, float] = {}
         batch: Optional[BatchCEM] = None
+        prev_tick: Optional[torch.cuda.Event] = None
         used = 0.0
         t0 = time.perf_counter()
         i = 0
-        while i < len(pending) or active or scoring:
+
+        def finish_ready(block: bool):
+            keep = []
+            for job_id, ev, host in inflight:
+                if block:
+                    ev.synchronize()
+                if ev.query():
+                    res = {k: v.numpy() for k, v in host.items()}
+                    if "best_cost" in res:
+                        res["best_cost"] = float(res["best_cost"][0])
+                    stats.results[job_id] = res
+                    stats.latency_s[job_id] = time.perf_counter() - t0 - arrived[job_id]
+                else:
+                    keep.append((job_id, ev, host))
+            inflight[:] = keep
+
+        while i < len(pending) or active or inflight:
             now = time.perf_counter() - t0
             t_adm = now if realtime else stats.ticks
             for r in pending[i:]:
@@ -111,11 +144,15 @@ class SysIdServer:
                     break
                 arrived.setdefault(r.job_id, r.arrival if realtime else now)
             changed = False
-            # admit in arrival order while the head of the queue fits
-            while i < len(pending) and pending[i].arrival <= t_adm:
+            while i < len(pending) and pending[i].arrival <= t_adm:      # admit in arrival order
                 r = pending[i]
                 if isinstance(r, ScoreRequest):
-                    scoring.append(self._launch_score(r))
+                    out = self._launch_score(r)
+                    ev = torch.cuda.Event()
+                    with torch.cuda.stream(self.score_stream):
+                        host = {"score": self._to_host(out)}
+                        ev.record(self.score_stream)
+                    inflight.append((r.job_id, ev, host))
                     i += 1
                     continue
                 need = _Buffers.footprint(r.spec)
@@ -131,42 +168,49 @@ class SysIdServer:
                                      unroll=self.unroll, buffers=[a.buf for a in active],
                                      iterations_done=[a.done for a in active])
                 batch.iterate()
+                tick = torch.cuda.Event()
+                tick.record(main)
                 for a in active:
                     a.done += 1
                 stats.ticks += 1
                 stats.max_active = max(stats.max_active, len(active))
-            torch.cuda.synchronize()
-            now = time.perf_counter() - t0
-            keep = []
-            for a in active:
-                if a.done >= a.req.iterations:
-                    b = a.buf
-                    stats.results[a.req.job_id] = {"mean": b.mean.cpu().numpy(), "std": b.std.cpu().numpy(),
-                                                   "best": b.best.cpu().numpy(),
-                                                   "best_cost": float(b.best_cost.item())}
-                    stats.latency_s[a.req.job_id] = time.perf_counter() - t0 - arrived[a.req.job_id]
-                    used -= _Buffers.footprint(a.req.spec)
-                else:
-                    keep.append(a)
-            if len(keep) != len(active):
-                batch = None
-            active = keep
-            for job_id, out in scoring:          # the synchronize above covers the scoring stream
-                stats.results[job_id] = {"score": out.cpu().numpy()}
-                stats.latency_s[job_id] = time.perf_counter() - t0 - arrived[job_id]
-            scoring = []
+                keep = []
+                for a in active:
+                    if a.done >= a.req.iterations:
+                        b = a.buf
+                        host = {"mean": self._to_host(b.mean), "std": self._to_host(b.std),
+                                "best": self._to_host(b.best), "best_cost": self._to_host(b.best_cost)}
+                        ev = torch.cuda.Event()
+                        ev.record(main)
+                        inflight.append((a.req.job_id, ev, host))
+                        used -= _Buffers.footprint(a.req.spec)
+                    else:
+                        keep.append(a)
+                if len(keep) != len(active):
+                    batch = None
+                active = keep
+                # one tick of run-ahead: wait for the previous tick before preparing the next
+                if prev_tick is not None:
+                    prev_tick.synchronize()
+                prev_tick = tick
+            finish_ready(block=not active and i >= len(pending))
             if not active and i < len(pending):
                 if realtime:
                     wait = pending[i].arrival - (time.perf_counter() - t0)
                     if wait > 0:
-                        time.sleep(min(wait, 0.01))
+                        time.sleep(min(wait, 0.002))
+                    finish_ready(block=False)
                 else:
+                    if inflight:
+                        finish_ready(block=True)
                     stats.ticks = max(stats.ticks, int(math.ceil(pending[i].arrival)))
+        torch.cuda.synchronize()
         stats.wall_s = time.perf_counter() - t0
         return stats
 
-    def _launch_score(self, r: ScoreRequest):
-        self.score_stream.wait_stream(torch.cuda.current_stream())
+    def _launch_score(self, r: ScoreRequest) -> torch.Tensor:
+        # request tensors were made before run() started; run() orders the scoring stream
+        # after them once, so scoring overlaps with the CEM iterations on the main stream
         with torch.cuda.stream(self.score_stream):
             if r.mode == "exact":
                 obj = FisherObjective(self.engine, list(r.explore) if r.explore else IDENTIFIED)
@@ -177,7 +221,7 @@ class SysIdServer:
                 out = obj(r.actions, r.init)
             else:
                 raise ValueError(f"unknown scoring mode {r.mode!r}")
-        return (r.job_id, out)
+        return out
 
 
 def poisson_arrivals(n: int, rate_hz: float, seed: int) -> np.ndarray:

@@ -35,17 +35,19 @@ def test_library_exports_every_header_symbol():
 def test_ctypes_structs_match_c_layout():
     """sizeof/offsetof of SpiSimArgs and SpiCostArgs as compiled by the host compiler."""
     from spi_amd._lib import SpiCostArgs, SpiSimArgs, SpiTangentSpec
+    from spi_amd.agents.commands import SpiCommandSpec
     from spi_amd.agents.batch import SpiBatchProblem
     src = r"""
     #include <cstdio>
     #include <cstddef>
     #include "spi_hip.h"
     int main() {
-      std::printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(SpiSimArgs),
+      std::printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu ", sizeof(SpiSimArgs),
                   offsetof(SpiSimArgs, effort), offsetof(SpiSimArgs, motor_model), offsetof(SpiSimArgs, state_limit),
                   sizeof(SpiCostArgs), offsetof(SpiCostArgs, w), sizeof(SpiBatchProblem),
                   offsetof(SpiBatchProblem, seed), offsetof(SpiBatchProblem, mean),
                   offsetof(SpiBatchProblem, best_cost), sizeof(SpiTangentSpec), offsetof(SpiTangentSpec, param));
+      std::printf("%zu %zu %zu\n", sizeof(SpiCommandSpec), offsetof(SpiCommandSpec, lo), offsetof(SpiCommandSpec, n_bezier));
     }
     """
     with tempfile.TemporaryDirectory() as d:
@@ -58,7 +60,8 @@ def test_ctypes_structs_match_c_layout():
     expect = [ctypes.sizeof(SpiSimArgs), SpiSimArgs.effort.offset, SpiSimArgs.motor_model.offset,
               SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset,
               ctypes.sizeof(SpiBatchProblem), SpiBatchProblem.seed.offset, SpiBatchProblem.mean.offset,
-              SpiBatchProblem.best_cost.offset, ctypes.sizeof(SpiTangentSpec), SpiTangentSpec.param.offset]
+              SpiBatchProblem.best_cost.offset, ctypes.sizeof(SpiTangentSpec), SpiTangentSpec.param.offset,
+              ctypes.sizeof(SpiCommandSpec), SpiCommandSpec.lo.offset, SpiCommandSpec.n_bezier.offset]
     assert got == expect
 
 
