@@ -324,6 +324,19 @@ int spi_command_sequences(const SpiCommandSpec* spec, const float* params_dev, i
 int spi_pack_windows(const float* states_dev, const float* actions_dev, int n_steps, int horizon, int first,
                      int n_windows, float* segments_dev, hipStream_t stream);
 
+/* Tracking cost and its exact gradient with respect to all 25 parameter rows for every
+ * (segment, candidate) pair, by reverse mode (adjoint) with checkpoints every
+ * checkpoint_every control steps: cost_dev [n_seg][n_cand] (layout of spi_rollout_cost),
+ * grad_dev [n_seg][n_cand][25] (+inf cost and zero gradient for diverged lanes).  work_dev:
+ * at least spi_adjoint_work_floats() floats of scratch, owned by the caller.  Same derivative
+ * as spi_rollout_cost_grad at about a tenth of its cost for 22 parameters; decimation and
+ * substeps <= 8. */
+long long spi_adjoint_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every);
+int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
+                             int n_cand, const float* segments_dev, int n_seg, int horizon, int checkpoint_every,
+                             float* work_dev, long long work_floats, float* cost_dev, float* grad_dev,
+                             hipStream_t stream);
+
 /* Excitations as Fourier series: actions_dev[e][t][j] = scale * sum_k coeff_dev[e][j][k] *
  * sin(2 pi (k + 1) t dt_ctrl / period), k < n_terms. */
 int spi_fourier_excitations(const float* coeff_dev, int n_exc, int n_terms, int horizon, float dt_ctrl, float period,

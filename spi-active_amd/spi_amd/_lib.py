@@ -122,7 +122,10 @@ def lib() -> C.CDLL:
         "spi_gn_step": [vp, vp, ip, ip, ip, vp, vp, vp, vp, vp],
         "spi_command_sequences": [vp, vp, ip, ip, vp, vp],
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],
+        "spi_rollout_cost_adjoint": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
     }
+    L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
+    L.spi_adjoint_work_floats.restype = C.c_longlong
     for name, args in sigs.items():
         fn = getattr(L, name)
         fn.argtypes = args

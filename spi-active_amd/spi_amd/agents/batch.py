@@ -68,20 +68,41 @@ def _validate(k: int, s: ProblemSpec) -> None:
         raise ValueError(f"problem {k}: seg_weights must be float32 [n_seg] on the device")
 
 
+_PRIORS: Dict[int, tuple] = {}
+
+
+def _prior_on_device(prior: Prior, dev) -> tuple:
+    """Device copies of a prior's bounds and initial moments, uploaded once per prior object."""
+    hit = _PRIORS.get(id(prior))
+    if hit is not None and hit[0] is prior and hit[1] == str(dev):
+        return hit[2]
+    f = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).contiguous()
+    t = (f(prior.mean), f(prior.std), f(prior.lo), f(prior.hi), f(prior.min_std))
+    _PRIORS[id(prior)] = (prior, str(dev), t)
+    return t
+
+
+def validate_spec(spec: ProblemSpec) -> None:
+    _validate(0, spec)
+
+
 class _Buffers:
     """Device state of one problem (CEM moments, samples, costs, best so far)."""
     def __init__(self, spec: ProblemSpec, dev):
         n, n_seg = spec.n_samples, spec.segments.shape[0]
-        f = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev).contiguous()
-        self.mean, self.std = f(spec.prior.mean).clone(), f(spec.prior.std).clone()
-        self.lo, self.hi, self.min_std = f(spec.prior.lo), f(spec.prior.hi), f(spec.prior.min_std)
+        mean, std, self.lo, self.hi, self.min_std = _prior_on_device(spec.prior, dev)
+        self.mean, self.std = mean.clone(), std.clone()
         self.theta = torch.empty((P_COUNT, n), dtype=torch.float32, device=dev)
         self.cost_seg = torch.empty((n_seg, n), dtype=torch.float32, device=dev)
         self.cost = torch.empty(n, dtype=torch.float32, device=dev)
-        self.best = self.mean.clone()
+        self.best = mean.clone()
         self.best_cost = torch.full((1,), float("inf"), dtype=torch.float32, device=dev)
         self.seg_w = (spec.seg_weights if spec.seg_weights is not None else
                       torch.full((n_seg,), 1.0 / n_seg, dtype=torch.float32, device=dev))
+        # device addresses in SpiBatchProblem field order (taken once)
+        self.ptrs = (spec.segments.data_ptr(), self.seg_w.data_ptr(), self.mean.data_ptr(), self.std.data_ptr(),
+                     self.lo.data_ptr(), self.hi.data_ptr(), self.min_std.data_ptr(), self.theta.data_ptr(),
+                     self.cost_seg.data_ptr(), self.cost.data_ptr(), self.best.data_ptr(), self.best_cost.data_ptr())
 
     @staticmethod
     def footprint(spec: ProblemSpec) -> int:
@@ -90,16 +111,47 @@ class _Buffers:
         return 4 * (P_COUNT * n + n_seg * n + n + 7 * P_COUNT + n_seg) + spec.segments.numel() * 4
 
 
+_TABLE_DTYPE = np.dtype(SpiBatchProblem)
+_PTR_FIELDS = ("segments", "seg_weight", "mean", "std", "lo", "hi", "min_std", "theta", "cost_seg", "cost_out",
+               "best", "best_cost")
+
+
+def pack_problem_table(specs, bufs, start_iter, cargs, block: int, dev):
+    n = len(specs)
+    arr = np.zeros(n, dtype=_TABLE_DTYPE)
+    ptrs = np.array([b.ptrs for b in bufs], dtype=np.uint64).reshape(n, len(_PTR_FIELDS))
+    for k, name in enumerate(_PTR_FIELDS):
+        arr[name] = ptrs[:, k]
+    n_seg = np.array([s.segments.shape[0] for s in specs], np.int64)
+    n_samp = np.array([s.n_samples for s in specs], np.int64)
+    arr["n_seg"] = n_seg
+    arr["horizon"] = [s.horizon for s in specs]
+    arr["cost"]["mode"] = cargs.mode
+    arr["cost"]["w"] = np.array(list(cargs.w), np.float32)
+    arr["n_samples"] = n_samp
+    arr["n_elite"] = [max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9))) for s in specs]
+    arr["alpha"] = [s.alpha for s in specs]
+    arr["iteration"] = np.asarray(start_iter, np.uint32)
+    arr["seed"] = np.array([s.seed for s in specs], np.uint64)
+    tasks = n_seg * ((n_samp + block - 1) // block)
+    prefix = np.concatenate([[0], np.cumsum(tasks)[:-1]]).astype(np.int64)
+    table = torch.from_numpy(arr.view(np.uint8).copy()).to(dev)
+    return table, torch.from_numpy(prefix).to(dev), int(tasks.sum())
+
+
 class BatchCEM:
     """A fixed set of problems solved together; ``iterate()`` runs one CEM iteration of all."""
 
     def __init__(self, engine: RolloutEngine, specs: Sequence[ProblemSpec], block: int = 256, unroll: int = 1,
                  blocks_per_cu: int = 4, n_cu: int = 256, buffers: Optional[Sequence["_Buffers"]] = None,
-                 iterations_done: Optional[Sequence[int]] = None):
+                 iterations_done: Optional[Sequence[int]] = None, validate: bool = True):
+        """validate=False skips the per-problem checks for specs a caller has already passed
+        through ``validate_spec`` (the server does so once per job at admission)."""
         if not specs:
             raise ValueError("no problems")
-        for k, s in enumerate(specs):
-            _validate(k, s)
+        if validate:
+            for k, s in enumerate(specs):
+                _validate(k, s)
         self.engine = engine
         self.dev = engine.device
         # longest horizon first: the persistent grid takes tickets in table order
@@ -118,25 +170,10 @@ class BatchCEM:
         self._build_table()
 
     def _build_table(self):
-        arr = (SpiBatchProblem * len(self.specs))()
-        prefix, tot = [], 0
-        for i, (s, b) in enumerate(zip(self.specs, self.bufs)):
-            p = arr[i]
-            p.segments, p.seg_weight = _lib.ptr(s.segments), _lib.ptr(b.seg_w)
-            p.n_seg, p.horizon = s.segments.shape[0], s.horizon
-            p.cost = self.engine.cargs
-            p.n_samples = s.n_samples
-            p.n_elite = max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9)))
-            p.alpha, p.iteration, p.seed = s.alpha, self.start_iter[i], s.seed
-            p.mean, p.std, p.lo, p.hi = _lib.ptr(b.mean), _lib.ptr(b.std), _lib.ptr(b.lo), _lib.ptr(b.hi)
-            p.min_std, p.theta, p.cost_seg = _lib.ptr(b.min_std), _lib.ptr(b.theta), _lib.ptr(b.cost_seg)
-            p.cost_out, p.best, p.best_cost = _lib.ptr(b.cost), _lib.ptr(b.best), _lib.ptr(b.best_cost)
-            prefix.append(tot)
-            tot += p.n_seg * ((s.n_samples + self.block - 1) // self.block)
-        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-        self.table = torch.from_numpy(raw.copy()).to(self.dev)
-        self.prefix = torch.tensor(prefix, dtype=torch.int64, device=self.dev)
-        self.n_tasks = tot
+        """The device table of SpiBatchProblem records, filled column-wise (numpy structured
+        array with the C layout), and the task prefix sums."""
+        self.table, self.prefix, self.n_tasks = pack_problem_table(self.specs, self.bufs, self.start_iter,
+                                                                   self.engine.cargs, self.block, self.dev)
 
     def device_iterations(self) -> List[int]:
         """Iteration counters as the device table holds them (table order)."""

@@ -134,15 +134,53 @@ class FisherObjective:
         return self.logdet(F, status, E)
 
 
+def adjoint_cost_grad(engine: RolloutEngine, theta: torch.Tensor, segments: torch.Tensor, horizon: int,
+                      checkpoint_every: int = 10):
+    """Cost [n_seg, n] and gradient [n_seg, n, 25] (all parameter rows) by the reverse-mode
+    kernel ``spi_rollout_cost_adjoint`` with checkpoints every ``checkpoint_every`` control steps."""
+    n = theta.shape[1]
+    n_seg = segments.shape[0]
+    if segments.shape[1] != segment_stride(horizon):
+        raise ValueError("segment stride does not match the horizon")
+    L = _lib.lib()
+    wf = int(L.spi_adjoint_work_floats(n, n_seg, horizon, checkpoint_every))
+    if wf < 0:
+        raise ValueError("bad sizes")
+    dev = theta.device
+    work = torch.empty(max(wf, 1), dtype=torch.float32, device=dev)
+    cost = torch.empty((n_seg, n), dtype=torch.float32, device=dev)
+    grad = torch.empty((n_seg, n, P_COUNT), dtype=torch.float32, device=dev)
+    rc = L.spi_rollout_cost_adjoint(C_ref(engine.args), C_ref(engine.cargs), _lib.ptr(theta), n, n,
+                                    _lib.ptr(segments), n_seg, horizon, checkpoint_every, _lib.ptr(work), wf,
+                                    _lib.ptr(cost), _lib.ptr(grad), _lib.stream())
+    _lib.check(rc, "spi_rollout_cost_adjoint")
+    return cost, grad
+
+
 class RolloutCost(torch.autograd.Function):
     """cost[c] = sum_s w_s cost_s(theta[:, c]) with the exact gradient of the discrete rollout
-    with respect to the parameter rows ``rows`` (other rows get zero gradient)."""
+    with respect to the parameter rows ``rows`` (other rows get zero gradient).
+
+    method "adjoint" (default): the reverse-mode kernel with checkpoints every
+    ``checkpoint_every`` control steps, which the forward pass runs together with the cost;
+    method "tangent": forward mode, one lane per parameter row."""
 
     @staticmethod
     def forward(ctx, theta: torch.Tensor, engine: RolloutEngine, segments: torch.Tensor, horizon: int,
-                seg_weights: torch.Tensor, rows: Sequence[int]):
+                seg_weights: torch.Tensor, rows: Sequence[int], method: str = "adjoint", checkpoint_every: int = 10):
         n = theta.shape[1]
         theta = theta.detach().contiguous()
+        if method == "adjoint":
+            cost, grad = adjoint_cost_grad(engine, theta, segments, horizon, checkpoint_every)
+            w = seg_weights.to(torch.float32)
+            total = (w[:, None] * cost).sum(0)
+            full = torch.zeros((P_COUNT, n), dtype=torch.float32, device=theta.device)
+            idx = torch.as_tensor(list(rows), dtype=torch.long, device=theta.device)
+            full[idx] = (w[:, None, None] * grad).sum(0).t()[idx]
+            ctx.save_for_backward(full)
+            return total
+        if method != "tangent":
+            raise ValueError(f"unknown method {method!r}")
         spec = _lib.tangent_spec(rows)
         n_seg = segments.shape[0]
         if segments.shape[1] != segment_stride(horizon):
@@ -164,4 +202,4 @@ class RolloutCost(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_out):
         (full,) = ctx.saved_tensors
-        return full * g_out[None, :], None, None, None, None, None
+        return full * g_out[None, :], None, None, None, None, None, None, None

@@ -24,8 +24,9 @@ import numpy as np
 import torch
 
 from .agents.active import FDFIMObjective
-from .agents.batch import BatchCEM, ProblemSpec, _Buffers
+from .agents.batch import BatchCEM, ProblemSpec, _Buffers, validate_spec
 from .agents.fisher import IDENTIFIED, FisherObjective
+from .params import P_COUNT
 from .rollout import RolloutEngine
 
 
@@ -89,12 +90,6 @@ class SysIdServer:
         self.block, self.unroll = block, unroll
         self.score_stream = torch.cuda.Stream()   # default priority, like every stream here
 
-    @staticmethod
-    def _to_host(t: torch.Tensor) -> torch.Tensor:
-        h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-        h.copy_(t, non_blocking=True)
-        return h
-
     def run(self, requests: Sequence, clock: str = "wall") -> ServiceStats:
         """Serve requests until all are done.
 
@@ -107,13 +102,15 @@ class SysIdServer:
         realtime = clock == "wall"
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:
-            if isinstance(r, SysIdRequest) and _Buffers.footprint(r.spec) > self.budget:
-                raise ValueError(f"job {r.job_id} needs {_Buffers.footprint(r.spec)} B, over the whole budget")
+            if isinstance(r, SysIdRequest):
+                validate_spec(r.spec)
+                if _Buffers.footprint(r.spec) > self.budget:
+                    raise ValueError(f"job {r.job_id} needs {_Buffers.footprint(r.spec)} B, over the whole budget")
         stats = ServiceStats()
         main = torch.cuda.current_stream()
         self.score_stream.wait_stream(main)       # inputs of every request are ready
         active: List[_Active] = []
-        inflight: List = []            # (job_id, event, host tensors dict)
+        inflight: List = []            # (entries, event, pinned host tensor); entry = (job_id, kind, offset, size, dtype)
         arrived: Dict[int, float] = {}
         batch: Optional[BatchCEM] = None
         prev_tick: Optional[torch.cuda.Event] = None
@@ -121,19 +118,36 @@ class SysIdServer:
         t0 = time.perf_counter()
         i = 0
 
+        def ship(parts, entries, stream, dtype=None):
+            """One device concatenation and one pinned copy for all results of a tick, ordered on
+            the stream that produced them."""
+            with torch.cuda.stream(stream):
+                dev = torch.cat([x.reshape(-1) if dtype is None else x.reshape(-1).to(dtype) for x in parts])
+                host = torch.empty(dev.shape, dtype=dev.dtype, pin_memory=True)
+                host.copy_(dev, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(stream)
+            inflight.append((entries, ev, host))
+
         def finish_ready(block: bool):
             keep = []
-            for job_id, ev, host in inflight:
+            for entries, ev, host in inflight:
                 if block:
                     ev.synchronize()
-                if ev.query():
-                    res = {k: v.numpy() for k, v in host.items()}
-                    if "best_cost" in res:
-                        res["best_cost"] = float(res["best_cost"][0])
+                if not ev.query():
+                    keep.append((entries, ev, host))
+                    continue
+                h = host.numpy()
+                t_done = time.perf_counter() - t0
+                for job_id, kind, o, n, dt in entries:
+                    v = h[o:o + n]
+                    if kind == "cem":
+                        res = {"mean": v[0:P_COUNT].copy(), "std": v[P_COUNT:2 * P_COUNT].copy(),
+                               "best": v[2 * P_COUNT:3 * P_COUNT].copy(), "best_cost": float(v[3 * P_COUNT])}
+                    else:
+                        res = {"score": v.astype(dt)}
                     stats.results[job_id] = res
-                    stats.latency_s[job_id] = time.perf_counter() - t0 - arrived[job_id]
-                else:
-                    keep.append((job_id, ev, host))
+                    stats.latency_s[job_id] = t_done - arrived[job_id]
             inflight[:] = keep
 
         while i < len(pending) or active or inflight:
@@ -144,15 +158,14 @@ class SysIdServer:
                     break
                 arrived.setdefault(r.job_id, r.arrival if realtime else now)
             changed = False
+            score_parts, score_entries, so = [], [], 0
             while i < len(pending) and pending[i].arrival <= t_adm:      # admit in arrival order
                 r = pending[i]
                 if isinstance(r, ScoreRequest):
                     out = self._launch_score(r)
-                    ev = torch.cuda.Event()
-                    with torch.cuda.stream(self.score_stream):
-                        host = {"score": self._to_host(out)}
-                        ev.record(self.score_stream)
-                    inflight.append((r.job_id, ev, host))
+                    score_parts.append(out)
+                    score_entries.append((r.job_id, "score", so, out.numel(), np.dtype(str(out.dtype).split(".")[-1])))
+                    so += out.numel()
                     i += 1
                     continue
                 need = _Buffers.footprint(r.spec)
@@ -162,11 +175,13 @@ class SysIdServer:
                 used += need
                 changed = True
                 i += 1
+            if score_parts:
+                ship(score_parts, score_entries, self.score_stream, torch.float64)
             if active:
                 if changed or batch is None:
                     batch = BatchCEM(self.engine, [a.req.spec for a in active], block=self.block,
                                      unroll=self.unroll, buffers=[a.buf for a in active],
-                                     iterations_done=[a.done for a in active])
+                                     iterations_done=[a.done for a in active], validate=False)
                 batch.iterate()
                 tick = torch.cuda.Event()
                 tick.record(main)
@@ -174,19 +189,18 @@ class SysIdServer:
                     a.done += 1
                 stats.ticks += 1
                 stats.max_active = max(stats.max_active, len(active))
-                keep = []
+                keep, parts, entries = [], [], []
                 for a in active:
                     if a.done >= a.req.iterations:
                         b = a.buf
-                        host = {"mean": self._to_host(b.mean), "std": self._to_host(b.std),
-                                "best": self._to_host(b.best), "best_cost": self._to_host(b.best_cost)}
-                        ev = torch.cuda.Event()
-                        ev.record(main)
-                        inflight.append((a.req.job_id, ev, host))
+                        parts += [b.mean, b.std, b.best, b.best_cost]
+                        entries.append((a.req.job_id, "cem", len(entries) * (3 * P_COUNT + 1), 3 * P_COUNT + 1,
+                                        np.float32))
                         used -= _Buffers.footprint(a.req.spec)
                     else:
                         keep.append(a)
-                if len(keep) != len(active):
+                if parts:
+                    ship(parts, entries, main)
                     batch = None
                 active = keep
                 # one tick of run-ahead: wait for the previous tick before preparing the next

@@ -159,3 +159,49 @@ def test_cost_gradient_matches_oracle_and_central_differences(gpu, sim, theta0):
     print("directional derivative rel err (median, max):", np.median(rel), rel.max())
     # FP32 central differences carry rounding noise of ~1e-6 relative / eps
     assert np.median(rel) < 5e-2 and np.corrcoef(fd, an)[0, 1] > 0.999
+
+
+def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0):
+    """Reverse-mode kernel (checkpoints every 4 control steps, H 10) against the forward-mode
+    kernel on 64 candidates x 4 segments and the FP64 oracle gradient on 3 candidates."""
+    import time
+    import spio
+    from spi_amd.agents.fisher import IDENTIFIED, RolloutCost, _rows, adjoint_cost_grad
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.model import load_model
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    segs = synthetic_segments(eng.simulate_np, ground_truth(prior, seed=12), sim, n_seg=4, horizon=10, seed=13)
+    seg_dev = eng.upload_segments(segs)
+    cand = _samples(prior, 64, 14)
+    theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
+    cost_a, grad_a = adjoint_cost_grad(eng, theta, seg_dev, 10, checkpoint_every=4)
+    ref_cost = eng.segment_costs(theta, 64, seg_dev, 10)
+    assert torch.allclose(cost_a, ref_cost, rtol=1e-5, atol=1e-7)
+    w = torch.full((4,), 0.25, device="cuda")
+    rows = _rows(IDENTIFIED)
+    ta = theta.clone().requires_grad_(True)
+    RolloutCost.apply(ta, eng, seg_dev, 10, w, rows, "adjoint", 4).sum().backward()
+    tt = theta.clone().requires_grad_(True)
+    RolloutCost.apply(tt, eng, seg_dev, 10, w, rows, "tangent").sum().backward()
+    ga, gt = ta.grad.cpu().numpy()[rows], tt.grad.cpu().numpy()[rows]
+    scale = np.abs(gt).max(1, keepdims=True) + 1e-12
+    err = np.abs(ga - gt) / scale
+    print("adjoint vs tangent, max error / row scale:", err.max())
+    assert err.max() < 2e-2
+    o = spio.Oracle(load_model(), sim)
+    g = grad_a.cpu().numpy()
+    for c in range(3):
+        cr, gr = o.cost_grad(cand[c], np.arange(25), segs.init, segs.actions, segs.targets, 0, eng.weights)
+        sc = np.abs(gr).max()
+        assert np.abs(g[:, c, :] - gr).max() < 2e-2 * sc, (c, np.abs(g[:, c, :] - gr).max() / sc)
+    # cost of the two ways to the same gradient
+    for method in ("adjoint", "tangent"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = theta.clone().requires_grad_(True)
+        RolloutCost.apply(x, eng, seg_dev, 10, w, rows, method, 4).sum().backward()
+        torch.cuda.synchronize()
+        print(method, "gradient of 64 x 4 x H10:", round((time.perf_counter() - t0) * 1e3, 2), "ms")

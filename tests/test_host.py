@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "spi_hip.h")
 
 def _header_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(spi_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|long long|const char\*)\s+(spi_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_exports_every_header_symbol():
@@ -147,3 +147,36 @@ def test_config_composition():
     assert sp.dt == 0.005 and sp.decimation == 4 and sp.substeps == 4
     assert np.allclose(sp.kp, 20.0) and np.allclose(sp.kd, 0.5)
     assert get_class("spi_amd.config.AttrDict").__name__ == "AttrDict"
+
+
+def test_problem_table_packing_matches_c_layout():
+    """The column-wise numpy packing of the batched-problem table equals record-by-record
+    ctypes packing of SpiBatchProblem (fake device addresses; no GPU needed)."""
+    import ctypes
+    import math
+    from types import SimpleNamespace
+
+    import torch
+    from spi_amd._lib import cost_args
+    from spi_amd.agents.batch import SpiBatchProblem, pack_problem_table
+    rng = np.random.default_rng(0)
+    specs, bufs = [], []
+    for k in range(5):
+        H, n, n_seg = int(rng.choice([5, 20])), int(rng.choice([512, 8192])), int(rng.integers(1, 40))
+        specs.append(SimpleNamespace(segments=torch.empty((n_seg, 37 + 49 * H)), horizon=H, n_samples=n,
+                                     elite_frac=0.1, alpha=0.25 * k / 5, seed=1000 + k))
+        bufs.append(SimpleNamespace(ptrs=tuple(int(x) for x in rng.integers(1, 2**47, size=12))))
+    ca = cost_args(0, [10, 5, 1, 1, 1, 0.05])
+    table, prefix, n_tasks = pack_problem_table(specs, bufs, [0, 3, 1, 7, 2], ca, 128, "cpu")
+    arr = (SpiBatchProblem * 5)()
+    tot = 0
+    for i, (s, b) in enumerate(zip(specs, bufs)):
+        p = arr[i]
+        (p.segments, p.seg_weight, p.mean, p.std, p.lo, p.hi, p.min_std, p.theta, p.cost_seg, p.cost_out, p.best,
+         p.best_cost) = b.ptrs
+        p.n_seg, p.horizon, p.cost, p.n_samples = s.segments.shape[0], s.horizon, ca, s.n_samples
+        p.n_elite = max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9)))
+        p.alpha, p.iteration, p.seed = s.alpha, [0, 3, 1, 7, 2][i], s.seed
+        assert prefix[i].item() == tot
+        tot += p.n_seg * ((s.n_samples + 127) // 128)
+    assert bytes(table.numpy()) == bytes(arr) and n_tasks == tot
