@@ -46,7 +46,8 @@ def parse():
     p.add_argument("--cpu-candidates", type=int, default=256)
     p.add_argument("--no-plain-baseline", action="store_true")
     p.add_argument("--refine", type=int, default=10, help="Levenberg-Marquardt iterations after CEM (0: none)")
-    p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "serve"],
+    p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve",
+                                                    "policy"],
                    help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
                         "64 batched uneven problems, Fisher scoring of excitations and the serving loop")
     p.add_argument("--requests", type=int, default=10000, help="serve: number of requests")

@@ -342,6 +342,52 @@ int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, con
 int spi_fourier_excitations(const float* coeff_dev, int n_exc, int n_terms, int horizon, float dt_ctrl, float period,
                             float scale, float* actions_dev, hipStream_t stream);
 
+/* ---- Policy-in-the-loop active exploration (active_sysid.py:527-600 evaluate_policy over
+ * active_sysid_openloop.py).  Envs are numbered group-major: env g * group + k, k = 0 the main
+ * env; states [n_env][37] in the root layout; actions [n_env][12]; int flags [n_env]. */
+
+/* Observation width of the go2_omni actor: 60 current terms + 14 history steps of each. */
+int spi_policy_obs_dim(void);
+
+/* One control step (LeggedRobotBase.step, legged_robot_base.py:169-209, with the motor model of
+ * active_sysid_openloop.py:174-187): envs with done_dev set get zero actions
+ * (active_sysid.py:556-558); applied_dev receives the zeroed, clipped actions; with sync_first
+ * the first torque of each aux env uses its main env's joint state (the reference's view-only
+ * sync, active_sysid_openloop.py:316-330); bad_dev[i] is set when env i diverges.
+ * states_in_dev and states_out_dev must not alias. */
+int spi_policy_step(const SpiSimArgs* sim, const float* params_dev, int ld, int n_env, int group,
+                    const float* states_in_dev, float* states_out_dev, const float* actions_dev, const int* done_dev,
+                    int sync_first, float* applied_dev, int* bad_dev, hipStream_t stream);
+
+/* Per group: gravity termination OR-ed over the group into done_dev of every member
+ * (legged_robot_base.py:336-339, active_sysid_openloop.py:259-272); acc_dev[g] += termination_rew
+ * for a terminated group, else ||X||_F^2 of the finite-difference states
+ * (active_sysid_openloop.py:402-426); group_bad_dev[g] set on divergence. */
+int spi_policy_reward(const float* states_dev, int n_group, int group, float delta, float grav_x, float grav_y,
+                      float termination_rew, int* done_dev, const int* bad_dev, float* acc_dev, int* group_bad_dev,
+                      hipStream_t stream);
+
+/* Gait clock (go2_omni.py:348-379) with commands step_now, observation (legged_robot_base.py:
+ * 503-527, 795-844; go2_omni.py:627-668) with commands step_next; commands_dev
+ * [n_group][n_cmd_steps][14]; gait_dev [n_env]; history_dev [n_env][14][60] newest first;
+ * obs_dev [n_env][spi_policy_obs_dim()]. */
+int spi_policy_obs(int n_env, int group, const float* states_dev, const float* commands_dev, int n_cmd_steps,
+                   int step_now, int step_next, float dt_ctrl, const float* q_default_dev, float* gait_dev,
+                   const float* applied_dev, float* history_dev, float* obs_dev, int evaluating, float height_lo,
+                   float height_hi, float obs_clip, hipStream_t stream);
+
+/* Full-state copy of each main env into its aux envs (a corrected sync; the reference's reaches
+ * only the next first torque, see spi_policy_step). */
+int spi_policy_sync_state(float* states_dev, int n_env, int group, hipStream_t stream);
+
+/* Actor MLP (modules.py:47-63, ppo_modules.py:73 act_inference): n_layers Linear layers of widths
+ * dims[0] -> ... -> dims[n_layers], ELU between them.  params_dev packs, per layer, W [out][in]
+ * (torch Linear layout) then b [out]; work_dev holds spi_mlp_work_floats() floats. */
+long long spi_mlp_param_floats(int n_layers, const int* dims);
+long long spi_mlp_work_floats(int M, int n_layers, const int* dims);
+int spi_mlp_forward(const float* x_dev, int M, int n_layers, const int* dims, const float* params_dev, float* work_dev,
+                    long long work_floats, float* y_dev, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

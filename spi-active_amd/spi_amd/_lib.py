@@ -123,9 +123,19 @@ def lib() -> C.CDLL:
         "spi_command_sequences": [vp, vp, ip, ip, vp, vp],
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],
         "spi_rollout_cost_adjoint": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
+        "spi_policy_step": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],
+        "spi_policy_reward": [vp, ip, ip, fp, fp, fp, fp, vp, vp, vp, vp, vp],
+        "spi_policy_obs": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, vp, ip, fp, fp, fp, vp],
+        "spi_policy_sync_state": [vp, ip, ip, vp],
+        "spi_mlp_forward": [vp, ip, ip, C.POINTER(ip), vp, vp, C.c_longlong, vp, vp],
     }
     L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
     L.spi_adjoint_work_floats.restype = C.c_longlong
+    L.spi_policy_obs_dim.restype = ip
+    for name in ("spi_mlp_param_floats", "spi_mlp_work_floats"):
+        getattr(L, name).restype = C.c_longlong
+    L.spi_mlp_param_floats.argtypes = [ip, C.POINTER(ip)]
+    L.spi_mlp_work_floats.argtypes = [ip, ip, C.POINTER(ip)]
     for name, args in sigs.items():
         fn = getattr(L, name)
         fn.argtypes = args

@@ -135,8 +135,16 @@ def pack_problem_table(specs, bufs, start_iter, cargs, block: int, dev):
     arr["seed"] = np.array([s.seed for s in specs], np.uint64)
     tasks = n_seg * ((n_samp + block - 1) // block)
     prefix = np.concatenate([[0], np.cumsum(tasks)[:-1]]).astype(np.int64)
-    table = torch.from_numpy(arr.view(np.uint8).copy()).to(dev)
-    return table, torch.from_numpy(prefix).to(dev), int(tasks.sum())
+    return _upload(arr.view(np.uint8), dev), _upload(prefix, dev), int(tasks.sum())
+
+
+def _upload(a: np.ndarray, dev) -> torch.Tensor:
+    """Host array -> device tensor without a stream synchronisation (pinned staging from the
+    caching host allocator, asynchronous copy on the current stream)."""
+    t = torch.from_numpy(np.ascontiguousarray(a).copy())
+    if torch.device(dev).type == "cuda":
+        return t.pin_memory().to(dev, non_blocking=True)
+    return t
 
 
 class BatchCEM:

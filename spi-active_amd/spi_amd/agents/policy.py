@@ -1,0 +1,324 @@
+"""Policy-in-the-loop active exploration: a locomotion policy tracks each sampled command sequence
+while the finite-difference Fisher information of (1 main + P aux) env groups accumulates.
+
+Reference: ``ActiveSysId.evaluate_policy`` (spigym/agents/sysid/active_sysid.py:527-600) over
+``ActiveSysId_OpenLoop`` (spigym/envs/sysid/active_sysid_openloop.py) with the go2_omni actor
+(observation config/obs/loco/go2_omni.yaml, MLP 512-256-128 ELU, config/algo/ppo.yaml:32-47).
+Everything per step runs on the device: ``spi_mlp_forward`` (actor), ``spi_policy_step``
+(physics), ``spi_policy_reward`` (termination + FIM reward), ``spi_policy_obs`` (gait clock,
+observation, history).  The whole rollout can be captured once as a HIP graph and replayed.
+
+Step order (one reset step with zero actions, then total_steps - 2 policy steps,
+base_task.py:90-100 and active_sysid.py:539-576):
+  * the gait clock of a step uses the commands the step ran with, the observation the commands
+    after the playback pointer advanced (go2_omni.py:348-379, active_sysid_openloop.py:196-201);
+  * termination (gravity, group OR) and the FIM reward use the post-step state; a terminated
+    group earns ``termination_rew`` and gets zero actions at the next step;
+  * aux envs are synced to their main env after steps where step_idx % ksync == 1
+    (every step for ksync <= 1), step_idx counting the reset step as 1;
+  * the result is the sum of rewards over the policy steps divided by total_steps - 1.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ..params import PARAM_NAMES
+from ..rollout import C_ref, RolloutEngine
+from .active import KSYNC_MODES, _alias
+
+OBS_TERMS = 60
+OBS_HISTORY = 14
+ACTOR_HIDDEN = (512, 256, 128)
+
+
+class PolicyMLP:
+    """The actor network on the device: Linear layers with ELU between them, weights packed per
+    layer as W [out][in] then b [out] (the torch Linear layout)."""
+
+    def __init__(self, layers: Sequence[tuple], device="cuda"):
+        _lib.require_gpu()
+        self.device = torch.device(device)
+        dims = [int(layers[0][0].shape[1])]
+        parts = []
+        for W, b in layers:
+            W = torch.as_tensor(W, dtype=torch.float32)
+            b = torch.as_tensor(b, dtype=torch.float32)
+            if W.dim() != 2 or W.shape[1] != dims[-1] or b.shape != (W.shape[0],):
+                raise ValueError("layer shapes do not chain")
+            dims.append(int(W.shape[0]))
+            parts += [W.reshape(-1), b]
+        self.dims = dims
+        self._dims_c = (C.c_int * len(dims))(*dims)
+        self.n_layers = len(dims) - 1
+        n = _lib.lib().spi_mlp_param_floats(self.n_layers, self._dims_c)
+        if n < 0:
+            raise ValueError("unsupported layer stack")
+        self.params = torch.cat([p.cpu() for p in parts]).to(self.device).contiguous()
+        assert self.params.numel() == n
+        self._work: Optional[torch.Tensor] = None
+
+    @classmethod
+    def from_state_dict(cls, sd: Dict[str, torch.Tensor], prefix: str = "actor_module.module.", device="cuda"):
+        """From the reference's actor state dict (``loaded_dict["actor_model_state_dict"]``,
+        ppo.py:138-143): Linear layers at the even indices of the nn.Sequential."""
+        idx = sorted({int(k[len(prefix):].split(".")[0]) for k in sd if k.startswith(prefix) and k.endswith("weight")})
+        if not idx:
+            raise KeyError(f"no '{prefix}*.weight' entries in the state dict")
+        return cls([(sd[f"{prefix}{i}.weight"], sd[f"{prefix}{i}.bias"]) for i in idx], device)
+
+    @classmethod
+    def from_checkpoint(cls, path: str, device="cuda"):
+        """Load a reference PPO checkpoint (tensors only: weights_only=True)."""
+        ck = torch.load(path, map_location="cpu", weights_only=True)
+        return cls.from_state_dict(ck["actor_model_state_dict"], device=device)
+
+    @classmethod
+    def random(cls, in_dim: int, out_dim: int = 12, hidden: Sequence[int] = ACTOR_HIDDEN, seed: int = 0,
+               device="cuda"):
+        """Random weights with torch.nn.Linear's default initialisation."""
+        g = torch.Generator().manual_seed(seed)
+        dims = [in_dim, *hidden, out_dim]
+        layers = []
+        for i, o in zip(dims[:-1], dims[1:]):
+            bound = 1.0 / np.sqrt(i)
+            layers.append(((torch.rand(o, i, generator=g) * 2 - 1) * bound, (torch.rand(o, generator=g) * 2 - 1) * bound))
+        return cls(layers, device)
+
+    def layers(self):
+        """(W, b) per layer, views into the packed device buffer."""
+        out, off = [], 0
+        for i, o in zip(self.dims[:-1], self.dims[1:]):
+            W = self.params[off:off + o * i].view(o, i)
+            off += o * i
+            out.append((W, self.params[off:off + o]))
+            off += o
+        return out
+
+    def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        if x.dim() != 2 or x.shape[1] != self.dims[0]:
+            raise ValueError(f"input must be [M, {self.dims[0]}]")
+        M = x.shape[0]
+        L = _lib.lib()
+        need = L.spi_mlp_work_floats(M, self.n_layers, self._dims_c)
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(max(need, 1), dtype=torch.float32, device=self.device)
+        if out is None:
+            out = torch.empty((M, self.dims[-1]), dtype=torch.float32, device=self.device)
+        rc = L.spi_mlp_forward(_lib.ptr(x), M, self.n_layers, self._dims_c, _lib.ptr(self.params),
+                               _lib.ptr(self._work), self._work.numel(), _lib.ptr(out), _lib.stream())
+        _lib.check(rc, "spi_mlp_forward")
+        return out
+
+
+def reset_states(n_env: int, q_default: np.ndarray, seed: int = 0, base_height: float = 0.34,
+                 vel_range: float = 0.5) -> np.ndarray:
+    """Reset states drawn like the reference's reset of every env: joint positions
+    q_default * U[0.5, 1.5], zero joint velocities (active_sysid_openloop.py:274-292); base at
+    (0, 0, base_height), identity orientation, linear and angular velocity U[-0.5, 0.5]
+    (legged_robot_base.py:761-784, go2.yaml init_state).  [n_env, 37] float32."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros((n_env, 37), np.float32)
+    s[:, 2] = base_height
+    s[:, 6] = 1.0
+    s[:, 7:13] = rng.uniform(-vel_range, vel_range, size=(n_env, 6))
+    s[:, 13:25] = np.asarray(q_default, np.float32)[None] * rng.uniform(0.5, 1.5, size=(n_env, 12))
+    return s
+
+
+class PolicyFIMRollout:
+    """Scores command sequences by the FD Fisher information of a policy-driven rollout."""
+
+    def __init__(self, engine: RolloutEngine, policy: PolicyMLP, theta: np.ndarray, explore: Sequence[str],
+                 delta: float = 0.1, ksync_steps: int = 5, ksync_mode: str = "reference",
+                 termination_rew: float = 0.0, gravity_limits=(0.8, 0.8), height_limits=(-0.25, 0.15),
+                 obs_clip: float = 100.0, evaluating: bool = True):
+        self.engine = engine
+        self.policy = policy
+        obs_dim = _lib.lib().spi_policy_obs_dim()
+        if policy.dims[0] != obs_dim or policy.dims[-1] != 12:
+            raise ValueError(f"policy must map {obs_dim} observations to 12 actions")
+        self.obs_dim = obs_dim
+        self.theta = np.asarray(theta, np.float64)
+        self.idx = [PARAM_NAMES.index(p) if p in PARAM_NAMES else _alias(p) for p in explore]
+        self.P = len(self.idx)
+        self.G = self.P + 1
+        self.delta = float(delta)
+        self.ksync = int(ksync_steps)
+        self.ksync_mode = KSYNC_MODES[ksync_mode]
+        self.termination_rew = float(termination_rew)
+        self.grav = tuple(float(g) for g in gravity_limits)
+        self.height = tuple(float(h) for h in height_limits)
+        self.obs_clip = float(obs_clip)
+        self.evaluating = int(bool(evaluating))
+        sim = engine.sim
+        self.dt_ctrl = float(sim.dt * sim.decimation)
+        self.q_default = torch.as_tensor(np.asarray(sim.q0, np.float32), device=engine.device)
+        self._bufs: Optional[dict] = None
+        self._graph = None
+
+    # ------------------------------------------------------------------ layout
+    def params(self, n_group: int) -> torch.Tensor:
+        """SoA [25, n_env]: aux env k carries theta[idx[k-1]] + delta (active_sysid_openloop.py:62-68)."""
+        rows = np.repeat(self.theta[None], n_group * self.G, axis=0)
+        for k, p in enumerate(self.idx):
+            rows[k + 1::self.G, p] += self.delta
+        return torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).to(self.engine.device)
+
+    def sync_after(self, step_idx: int) -> bool:
+        """active_sysid_openloop.py:246-252."""
+        return self.ksync <= 1 or step_idx % self.ksync == 1
+
+    def _alloc(self, M: int, T: int) -> dict:
+        b = self._bufs
+        if b is not None and b["M"] == M and b["T"] == T:
+            return b
+        dev, n = self.engine.device, M * self.G
+        f = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
+        i = lambda *s: torch.zeros(s, dtype=torch.int32, device=dev)
+        b = {"M": M, "T": T, "n": n, "par": self.params(M), "s": [f(n, 37), f(n, 37)], "init": f(n, 37),
+             "actions": f(n, 12), "applied": f(n, 12), "done": i(n), "bad": i(n), "gait": f(n),
+             "hist": f(n, OBS_HISTORY, OBS_TERMS), "obs": f(n, self.obs_dim), "acc": f(M), "gbad": i(M),
+             "cmds": f(M, T, 14)}
+        self._bufs = b
+        self._graph = None
+        return b
+
+    # ------------------------------------------------------------------ the rollout
+    def _run(self, b: dict) -> None:
+        """Every launch of one rollout from b['init'] and b['cmds'] (capturable)."""
+        L, st = _lib.lib(), _lib.stream()
+        n, M, T, G = b["n"], b["M"], b["T"], self.G
+        a = C_ref(self.engine.args)
+        par = b["par"]
+        s = b["s"]
+        s[0].copy_(b["init"])
+        for k in ("actions", "done", "bad", "gait", "hist", "acc", "gbad"):
+            b[k].zero_()
+
+        def physics(cur: int, sync_first: bool) -> None:
+            rc = L.spi_policy_step(a, _lib.ptr(par), par.shape[1], n, G, _lib.ptr(s[cur]), _lib.ptr(s[1 - cur]),
+                                   _lib.ptr(b["actions"]), _lib.ptr(b["done"], torch.int32), int(sync_first),
+                                   _lib.ptr(b["applied"]), _lib.ptr(b["bad"], torch.int32), st)
+            _lib.check(rc, "spi_policy_step")
+
+        def observe(cur: int, now: int) -> None:
+            rc = L.spi_policy_obs(n, G, _lib.ptr(s[cur]), _lib.ptr(b["cmds"]), T, now, now + 1, self.dt_ctrl,
+                                  _lib.ptr(self.q_default), _lib.ptr(b["gait"]), _lib.ptr(b["applied"]),
+                                  _lib.ptr(b["hist"]), _lib.ptr(b["obs"]), self.evaluating, self.height[0],
+                                  self.height[1], self.obs_clip, st)
+            _lib.check(rc, "spi_policy_obs")
+
+        def sync(cur: int, step_idx: int) -> bool:
+            if not self.sync_after(step_idx):
+                return False
+            if self.ksync_mode == KSYNC_MODES["state"]:
+                _lib.check(L.spi_policy_sync_state(_lib.ptr(s[cur]), n, G, st), "spi_policy_sync_state")
+                return False
+            return True
+
+        # reset step: zero actions, no reward (base_task.py:90-100)
+        cur = 0
+        physics(cur, False)
+        cur = 1
+        observe(cur, 0)
+        pending = sync(cur, 1)
+        for it in range(1, T - 1):
+            self.policy(b["obs"], out=b["actions"])
+            physics(cur, pending)
+            cur = 1 - cur
+            rc = L.spi_policy_reward(_lib.ptr(s[cur]), M, G, self.delta, self.grav[0], self.grav[1],
+                                     self.termination_rew, _lib.ptr(b["done"], torch.int32),
+                                     _lib.ptr(b["bad"], torch.int32), _lib.ptr(b["acc"]),
+                                     _lib.ptr(b["gbad"], torch.int32), st)
+            _lib.check(rc, "spi_policy_reward")
+            observe(cur, it)
+            pending = sync(cur, it + 1)
+        b["final"] = s[cur]
+
+    def evaluate(self, commands: torch.Tensor, init: torch.Tensor, graph: bool = False) -> torch.Tensor:
+        """commands [M, total_steps, 14] (one sequence per main env); init [M * (P + 1), 37] (or
+        [37]).  Returns the average per-step reward of each group's main env [M] (the value
+        active_sysid.py:210-213 tells the study, negated); NaN where a group diverged."""
+        if commands.dim() != 3 or commands.shape[2] != 14 or commands.shape[1] < 3:
+            raise ValueError("commands must be [M, total_steps >= 3, 14]")
+        M, T = commands.shape[0], commands.shape[1]
+        b = self._alloc(M, T)
+        if init.dim() == 1:
+            init = init[None].expand(b["n"], 37)
+        if init.shape != (b["n"], 37):
+            raise ValueError(f"init must be [{b['n']}, 37]")
+        b["cmds"].copy_(commands)
+        b["init"].copy_(init)
+        if graph:
+            if self._graph is None:
+                self._graph = self._capture(b)
+            self._graph.replay()
+        else:
+            self._run(b)
+        fim = b["acc"] / float(T - 1)
+        return torch.where(b["gbad"] != 0, torch.full_like(fim, float("nan")), fim)
+
+    def _capture(self, b: dict):
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            self._run(b)            # warm-up outside the graph (allocations, first launches)
+        torch.cuda.current_stream().wait_stream(s)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._run(b)
+        return g
+
+    def final_states(self) -> torch.Tensor:
+        return self._bufs["final"]
+
+    def flags(self):
+        """(done [n_env], bad [n_env], group_bad [M]) after the last evaluation."""
+        b = self._bufs
+        return b["done"], b["bad"], b["gbad"]
+
+
+class PolicyActiveSysId:
+    """``ActiveSysId.optimize`` (active_sysid.py:165-257) with every stage on the device: each
+    iteration samples ``batch_size`` trials of command parameters (device CEM, the role of the
+    Optuna sampler), expands them into command sequences (``CommandGenerator``, the role of
+    ``sample_commands``), scores them with a policy-driven FIM rollout (``evaluate_policy``) and
+    updates the CEM with the negated FIM (``tell(-total_reward)``).  The host only reads the
+    best trial at the end."""
+
+    def __init__(self, rollout: PolicyFIMRollout, generator, batch_size: int = 1024, elite_frac: float = 0.1,
+                 seed: int = 0, init: Optional[torch.Tensor] = None, graph: bool = True):
+        from .cem import CEMConfig, DeviceCEM
+        self.rollout, self.gen, self.graph = rollout, generator, graph
+        bounds = np.asarray(generator.param_bounds(), np.float64)
+        lo, hi = bounds[:, 0], bounds[:, 1]
+        self.names = generator.param_names()
+        self.cem = DeviceCEM(0.5 * (lo + hi), 0.5 * (hi - lo), lo, hi, np.maximum(1e-3 * (hi - lo), 0.0),
+                             CEMConfig(n_samples=batch_size, elite_frac=elite_frac, seed=seed))
+        if init is None:
+            n_env = batch_size * rollout.G
+            init = torch.from_numpy(reset_states(n_env, rollout.engine.sim.q0, seed=seed)).to(rollout.engine.device)
+        self.init = init
+        self.history = []
+
+    def step(self) -> torch.Tensor:
+        theta = self.cem.ask()                                 # [n_params, B]
+        cmds = self.gen.generate(theta.t())                    # [B, total_steps, 14]
+        fim = self.rollout.evaluate(cmds, self.init, graph=self.graph)
+        cost = torch.nan_to_num(-fim, nan=float("inf"))
+        self.cem.tell(cost)
+        self.history.append(fim)
+        return fim
+
+    def optimize(self, iterations: int) -> Dict:
+        for _ in range(iterations):
+            self.step()
+        best = self.cem.best.cpu().numpy().astype(np.float64)
+        params = dict(zip(self.names, best.tolist()))
+        return {"best_params": params, "best_value": float(self.cem.best_cost.item()),
+                "best_commands": self.gen.from_params(params)}

@@ -64,13 +64,15 @@ class ServiceStats:
     ticks: int = 0
     max_active: int = 0
     wall_s: float = 0.0
+    host_s: float = 0.0            # host time spent admitting, rebuilding tables and launching
+    wait_s: float = 0.0            # host time blocked on the previous tick
 
     def summary(self) -> Dict:
         lat = np.array(list(self.latency_s.values())) if self.latency_s else np.zeros(1)
         return {"completed": len(self.latency_s), "p50_ms": float(np.percentile(lat, 50) * 1e3),
                 "p99_ms": float(np.percentile(lat, 99) * 1e3),
                 "jobs_per_s": len(self.latency_s) / max(self.wall_s, 1e-9), "ticks": self.ticks,
-                "max_active": self.max_active, "wall_s": self.wall_s}
+                "max_active": self.max_active, "wall_s": self.wall_s, "host_s": self.host_s, "wait_s": self.wait_s}
 
 
 class SysIdServer:
@@ -106,6 +108,13 @@ class SysIdServer:
                 validate_spec(r.spec)
                 if _Buffers.footprint(r.spec) > self.budget:
                     raise ValueError(f"job {r.job_id} needs {_Buffers.footprint(r.spec)} B, over the whole budget")
+        # device copies of the scoring parameters, made once before serving starts (a pageable
+        # upload inside the loop would wait for the scoring stream)
+        self._theta_dev = {}
+        for r in pending:
+            if isinstance(r, ScoreRequest) and id(r.theta) not in self._theta_dev:
+                self._theta_dev[id(r.theta)] = torch.as_tensor(np.asarray(r.theta, np.float32).reshape(-1, 1),
+                                                               device=self.engine.device).contiguous()
         stats = ServiceStats()
         main = torch.cuda.current_stream()
         self.score_stream.wait_stream(main)       # inputs of every request are ready
@@ -151,7 +160,8 @@ class SysIdServer:
             inflight[:] = keep
 
         while i < len(pending) or active or inflight:
-            now = time.perf_counter() - t0
+            th0 = time.perf_counter()
+            now = th0 - t0
             t_adm = now if realtime else stats.ticks
             for r in pending[i:]:
                 if r.arrival > t_adm:
@@ -204,8 +214,11 @@ class SysIdServer:
                     batch = None
                 active = keep
                 # one tick of run-ahead: wait for the previous tick before preparing the next
+                tw = time.perf_counter()
+                stats.host_s += tw - th0
                 if prev_tick is not None:
                     prev_tick.synchronize()
+                stats.wait_s += time.perf_counter() - tw
                 prev_tick = tick
             finish_ready(block=not active and i >= len(pending))
             if not active and i < len(pending):
@@ -228,8 +241,11 @@ class SysIdServer:
         with torch.cuda.stream(self.score_stream):
             if r.mode == "exact":
                 obj = FisherObjective(self.engine, list(r.explore) if r.explore else IDENTIFIED)
-                th = torch.as_tensor(np.asarray(r.theta, np.float32).reshape(-1, 1), device=self.engine.device)
-                out = obj.score(th.contiguous(), r.actions, r.init).logdet
+                th = self._theta_dev.get(id(r.theta))
+                if th is None:
+                    th = torch.as_tensor(np.asarray(r.theta, np.float32).reshape(-1, 1),
+                                         device=self.engine.device).contiguous()
+                out = obj.score(th, r.actions, r.init).logdet
             elif r.mode == "fd":
                 obj = FDFIMObjective(self.engine, r.theta, list(r.explore or ("mass",)), delta=r.delta)
                 out = obj(r.actions, r.init)

@@ -278,4 +278,139 @@ def serve(n_requests: int = 10000, rate_hz: float = 200.0, budget_gb: float = 20
                        "score_horizon": 50, "seed": seed},
             "latency_ms": {"p50": round(s["p50_ms"], 2), "p99": round(s["p99_ms"], 2), "by_kind": by},
             "sysid_solves_per_s": round(n_sysid / s["wall_s"], 2), "wall_s": round(s["wall_s"], 2),
-            "offered_work_control_steps": int(work), "ticks": s["ticks"], "max_active": s["max_active"]}
+            "offered_work_control_steps": int(work), "ticks": s["ticks"], "max_active": s["max_active"],
+            "host_s": round(s["host_s"], 2), "wait_s": round(s["wait_s"], 2)}
+
+
+# ------------------------------------------------------------------ exact gradients
+def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, horizon: int = 20,
+             checkpoint_every: int = 10) -> Dict:
+    """Gradient of the tracking cost with respect to all parameters for every (sample, segment)
+    rollout of the 22-parameter CEM workload: reverse mode (checkpoints every 10 control steps)
+    timed over the full 4096 x 256 x H 20, forward mode (22 tangent lanes) on a 1/16 slice."""
+    from .agents.fisher import IDENTIFIED, _rows, adjoint_cost_grad
+    from . import _lib
+    from .rollout import C_ref
+    sim = SimParams()
+    eng = RolloutEngine(sim)
+    prior = identification_prior()
+    segs = synthetic_segments(eng.simulate_np, ground_truth(prior, seed=1), sim, n_seg=n_seg, horizon=horizon, seed=2)
+    seg_dev = eng.upload_segments(segs)
+    rng = np.random.default_rng(0)
+    rows = np.repeat(prior.mean[None], n_samples, axis=0)
+    rows[:, prior.free] += rng.normal(size=(n_samples, int(prior.free.sum()))) * prior.std[prior.free]
+    rows = np.clip(rows, prior.lo, prior.hi)
+    theta = torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).cuda()
+    st = torch.cuda.current_stream()
+    ms = []
+    for rep in range(warmup + steps):
+        a, b = _events(st)
+        a.record(st)
+        cost, grad = adjoint_cost_grad(eng, theta, seg_dev, horizon, checkpoint_every)
+        b.record(st)
+        torch.cuda.synchronize()
+        if rep >= warmup:
+            ms.append(a.elapsed_time(b))
+        del grad
+    adj_ms = float(np.mean(ms))
+    # primal for the ratio
+    a, b = _events(st)
+    a.record(st)
+    eng.segment_costs(theta, n_samples, seg_dev, horizon)
+    b.record(st)
+    torch.cuda.synchronize()
+    primal_ms = a.elapsed_time(b)
+    # forward mode on a slice of the segments (22 lanes per rollout)
+    spec = _lib.tangent_spec(_rows(IDENTIFIED))
+    n_sl = max(1, n_seg // 16)
+    c2 = torch.empty((n_sl, n_samples), dtype=torch.float32, device="cuda")
+    g2 = torch.empty((n_sl, n_samples, spec.n_tan), dtype=torch.float32, device="cuda")
+    a, b = _events(st)
+    a.record(st)
+    rc = _lib.lib().spi_rollout_cost_grad(C_ref(eng.args), C_ref(eng.cargs), C_ref(spec), _lib.ptr(theta),
+                                          n_samples, n_samples, _lib.ptr(seg_dev[:n_sl].contiguous()), n_sl, horizon,
+                                          _lib.ptr(c2), _lib.ptr(g2), _lib.stream())
+    _lib.check(rc, "spi_rollout_cost_grad")
+    b.record(st)
+    torch.cuda.synchronize()
+    tan_ms_full = a.elapsed_time(b) * (n_seg / n_sl)
+    rollouts = n_samples * n_seg
+    return {"metric": "gradient rollouts/s (d cost / d theta, 25 rows, reverse mode)",
+            "value": round(rollouts / (adj_ms / 1e3), 1), "unit": "rollouts/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": round(adj_ms, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": "gradient_4096x256_h20", "samples": n_samples, "segments": n_seg,
+                       "horizon": horizon, "checkpoint_every": checkpoint_every},
+            "adjoint_ms": round(adj_ms, 2), "primal_ms": round(primal_ms, 2),
+            "adjoint_over_primal": round(adj_ms / primal_ms, 2),
+            "forward_mode_ms_extrapolated": round(tan_ms_full, 1),
+            "forward_mode_over_adjoint": round(tan_ms_full / adj_ms, 2)}
+
+
+def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, total_steps: int = 1250) -> Dict:
+    """Policy-in-the-loop active exploration (run_active_sysid.py end to end on the device): each
+    iteration samples ``batch`` command trials (constant mode, dims 0, 2, 5, 5 windows), expands
+    them into 1250-step command sequences, rolls out (1 main + 1 aux) env groups driven by the
+    go2_omni actor (900 -> 512 -> 256 -> 128 -> 12, ELU; random weights) and updates the CEM with
+    the negated FIM.  Timed at the reference's batch (1024 trials) and at a wide batch that
+    fills the GPU; the actor's share is measured separately on the same rows."""
+    from .agents.commands import CommandGenerator
+    from .agents.policy import PolicyActiveSysId, PolicyFIMRollout, PolicyMLP
+    import copy
+    sim = copy.deepcopy(SimParams())
+    sim.control_mode, sim.motor_model = 1, 3
+    eng = RolloutEngine(sim)
+    theta = nominal_params(load_model())
+    ranges = [[-1.0, 1.0], [-1.0, 1.0], [-1.0, 1.0], [-0.25, 0.15], [2.0, 4.0], [0.0, 1.0], [0.0, 1.0], [0.0, 1.0],
+              [0.5, 0.5], [0.03, 0.035], [-0.4, 0.4], [-0.0, 0.0], [0.1, 0.45], [0.35, 0.45]]
+    default = [0.5, 0.5, 0.5, 0.0, 3.0, 0.0, 0.5, 0.5, 0.5, 0.03, 0.0, 0.0, 0.3, 0.4]
+    dt_ctrl = sim.dt * sim.decimation
+    gen = CommandGenerator(ranges, default, [0, 2, 5], "constant", rollout_length=total_steps * dt_ctrl,
+                           horizon_length=total_steps * dt_ctrl / 5, dt=dt_ctrl)
+    net = PolicyMLP.random(900, 12, seed=0)
+    st = torch.cuda.current_stream()
+    res = {}
+    for name, B in (("reference_batch", batch), ("wide_batch", wide_batch)):
+        roll = PolicyFIMRollout(eng, net, theta, ["mass"], delta=0.1, ksync_steps=5)
+        drv = PolicyActiveSysId(roll, gen, batch_size=B, seed=1)
+        for _ in range(warmup):
+            drv.step()
+        torch.cuda.synchronize()
+        a, b = _events(st)
+        a.record(st)
+        for _ in range(steps):
+            drv.step()
+        b.record(st)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        n_env = B * roll.G
+        T = gen.total_steps
+        # actor share: the MLP alone on the same rows, replayed per control step
+        obs = torch.randn((n_env, 900), device="cuda")
+        out = torch.empty((n_env, 12), device="cuda")
+        net(obs, out=out)
+        a2, b2 = _events(st)
+        a2.record(st)
+        for _ in range(50):
+            net(obs, out=out)
+        b2.record(st)
+        torch.cuda.synchronize()
+        mlp_ms = a2.elapsed_time(b2) / 50
+        flops = 2.0 * sum(i * o for i, o in zip(net.dims[:-1], net.dims[1:])) * n_env
+        fim = drv.history[-1]
+        res[name] = {"trials": B, "envs": n_env, "control_steps": T - 1, "ms_per_iteration": round(ms, 2),
+                     "env_steps_per_s": round(n_env * (T - 1) / (ms / 1e3), 1),
+                     "trials_per_s": round(B / (ms / 1e3), 1),
+                     "actor_ms_per_control_step": round(mlp_ms, 4),
+                     "actor_share": round(mlp_ms * (T - 2) / ms, 3),
+                     "actor_tflops": round(flops / (mlp_ms / 1e3) / 1e12, 2),
+                     "actor_frac_of_fp32_mfma_peak": round(flops / (mlp_ms / 1e3) / 1e12 / PEAK_FP32_TFLOPS, 3),
+                     "finite_fraction": round(float(torch.isfinite(fim).float().mean().item()), 4),
+                     "best_value": float(drv.cem.best_cost.item())}
+    ref = res["reference_batch"]
+    return {"metric": "policy-in-the-loop env-steps/s (active exploration, 1250-step rollouts)",
+            "value": ref["env_steps_per_s"], "unit": "env-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ref["ms_per_iteration"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32", "data": "synthetic (random actor weights)",
+            "config": {"workload": "policy_active_sysid", "explore": ["mass"], "total_steps": total_steps,
+                       "command_mode": "constant", "ksync_steps": 5}, **res}
