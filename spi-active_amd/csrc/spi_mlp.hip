@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 
 #include "spi_types.h"
@@ -24,8 +25,10 @@ constexpr int kMaxLayers = 8;
 // Y[m][n] = act(sum_k X[m][k] W[n][k] + b[n]); W row-major [N][K] (torch Linear layout).
 // MFMA operands: lane l supplies A[l % 32][k + l / 32] = X-tile element and
 // B[k + l / 32][l % 32] = W[n0 + l % 32][k + l / 32]; accumulator r of lane l holds
-// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
-template <bool kElu>
+// D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  The next K chunk is fetched from global memory
+// into registers while the MFMAs consume the current one from LDS.  kVec: rows of X and W are
+// 16-byte aligned and K is a multiple of 4, so each thread moves float4s.
+template <bool kElu, bool kVec>
 __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X, int ldx, int M, int K,
                                                      const float* __restrict__ W, const float* __restrict__ b, int N,
                                                      float* __restrict__ Y, int ldy) {
@@ -38,20 +41,49 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
   floatx16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  constexpr int kV = kVec ? 4 : 1;                     // floats per load
+  constexpr int kTPR = kKc / kV;                       // threads per tile row
+  constexpr int kRowStep = 256 / kTPR;                 // rows covered by one pass of the block
+  constexpr int kAPer = kRows / kRowStep, kWPer = kCols / kRowStep;
+  const int r0 = tid / kTPR, c0 = (tid % kTPR) * kV;
+  float ra[kAPer][kV], rw[kWPer][kV];
+  auto fetch = [&](int k0) {
+    const int gk = k0 + c0;
+#pragma unroll
+    for (int i = 0; i < kAPer; ++i) {
+      const int gm = m0 + r0 + i * kRowStep;
+      if (kVec) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (gm < M && gk < K) v = *reinterpret_cast<const float4*>(X + static_cast<size_t>(gm) * ldx + gk);
+        ra[i][0] = v.x; ra[i][kV > 1 ? 1 : 0] = v.y; ra[i][kV > 2 ? 2 : 0] = v.z; ra[i][kV > 3 ? 3 : 0] = v.w;
+      } else {
+        ra[i][0] = (gm < M && gk < K) ? X[static_cast<size_t>(gm) * ldx + gk] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kWPer; ++i) {
+      const int gn = n0 + r0 + i * kRowStep;
+      if (kVec) {
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (gn < N && gk < K) v = *reinterpret_cast<const float4*>(W + static_cast<size_t>(gn) * K + gk);
+        rw[i][0] = v.x; rw[i][kV > 1 ? 1 : 0] = v.y; rw[i][kV > 2 ? 2 : 0] = v.z; rw[i][kV > 3 ? 3 : 0] = v.w;
+      } else {
+        rw[i][0] = (gn < N && gk < K) ? W[static_cast<size_t>(gn) * K + gk] : 0.0f;
+      }
+    }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < K; k0 += kKc) {
 #pragma unroll
-    for (int e = tid; e < kRows * kKc; e += 256) {
-      const int r = e / kKc, c = e % kKc;
-      const int gm = m0 + r, gk = k0 + c;
-      As[r * kPad + c] = (gm < M && gk < K) ? X[static_cast<size_t>(gm) * ldx + gk] : 0.0f;
-    }
+    for (int i = 0; i < kAPer; ++i)
 #pragma unroll
-    for (int e = tid; e < kCols * kKc; e += 256) {
-      const int r = e / kKc, c = e % kKc;
-      const int gn = n0 + r, gk = k0 + c;
-      Ws[r * kPad + c] = (gn < N && gk < K) ? W[static_cast<size_t>(gn) * K + gk] : 0.0f;
-    }
+      for (int v = 0; v < kV; ++v) As[(r0 + i * kRowStep) * kPad + c0 + v] = ra[i][v];
+#pragma unroll
+    for (int i = 0; i < kWPer; ++i)
+#pragma unroll
+      for (int v = 0; v < kV; ++v) Ws[(r0 + i * kRowStep) * kPad + c0 + v] = rw[i][v];
     __syncthreads();
+    if (k0 + kKc < K) fetch(k0 + kKc);
     if (live) {
       const float* ap = As + col * kPad + half;
       const float* bp = Ws + (wave * 32 + col) * kPad + half;
@@ -83,10 +115,19 @@ int mfail(int code, const char* msg) {
 int launch_linear(const float* X, int ldx, int M, int K, const float* W, const float* b, int N, float* Y, int ldy,
                   bool elu, hipStream_t stream) {
   const dim3 grid((M + kRows - 1) / kRows, (N + kCols - 1) / kCols);
-  if (elu)
-    linear_kernel<true><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
-  else
-    linear_kernel<false><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
+  const bool vec = K % 4 == 0 && ldx % 4 == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
+                   reinterpret_cast<uintptr_t>(W) % 16 == 0;
+  if (vec) {
+    if (elu)
+      linear_kernel<true, true><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
+    else
+      linear_kernel<false, true><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
+  } else {
+    if (elu)
+      linear_kernel<true, false><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
+    else
+      linear_kernel<false, false><<<grid, 256, 0, stream>>>(X, ldx, M, K, W, b, N, Y, ldy);
+  }
   const hipError_t e = hipGetLastError();
   if (e == hipSuccess) return SPI_OK;
   char buf[256];

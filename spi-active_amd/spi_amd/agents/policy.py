@@ -322,3 +322,67 @@ class PolicyActiveSysId:
         params = dict(zip(self.names, best.tolist()))
         return {"best_params": params, "best_value": float(self.cem.best_cost.item()),
                 "best_commands": self.gen.from_params(params)}
+
+
+# params_dict keys of the active-sysID env (env/active_sysid_openloop.yaml default_param) -> parameter
+# rows; "inertiay" has no setter in the reference (isaacgym_active_sysid.py:86 defines
+# set_inertiaiy), so it is applied only with apply_inertiay.
+def _param_rows(apply_inertiay: bool):
+    from ..params import P_COMX, P_COMY, P_COMZ, P_IXX, P_IYY, P_IZZ, P_MASS, P_MOTOR0
+    rows = {"mass": P_MASS, "comx": P_COMX, "comy": P_COMY, "comz": P_COMZ, "inertiax": P_IXX, "inertiaz": P_IZZ,
+            "motor_model_hip_a": P_MOTOR0, "motor_model_thigh_a": P_MOTOR0 + 1, "motor_model_calf_a": P_MOTOR0 + 2}
+    if apply_inertiay:
+        rows["inertiay"] = P_IYY
+    return rows
+
+
+def theta_from_default_param(default_param: Dict, nominal: Optional[np.ndarray] = None,
+                             apply_inertiay: bool = False) -> np.ndarray:
+    """The 25-row parameter vector of every env before exploration offsets: the nominal model with
+    each ``default_param`` entry (``[body, value]`` or ``[value]``) written to its row."""
+    from ..params import nominal_params
+    theta = np.array(nominal_params() if nominal is None else nominal, np.float64)
+    rows = _param_rows(apply_inertiay)
+    for key, spec in default_param.items():
+        spec = list(spec)
+        if len(spec) == 2 and spec[0] != "base":
+            raise NotImplementedError(f"default_param {key!r}: only the base body is identified")
+        if key in rows:
+            theta[rows[key]] = float(spec[-1])
+    return theta
+
+
+def active_sysid_from_config(cfg: Dict, policy: PolicyMLP, engine: Optional[RolloutEngine] = None,
+                             seed: int = 0) -> PolicyActiveSysId:
+    """Builds the device optimizer from the reference's composed config (num_envs,
+    env.config.{delta_param, ksync_steps, default_param, exploration_params},
+    algo.config.command.*, algo.config.optimize.sampler.batch_size), as ``run_active_sysid.py``
+    instantiates env and algo from it.  The batch is num_envs / (P + 1) trials."""
+    import copy
+    from ..params import SimParams
+    from .commands import CommandGenerator
+    env, algo = cfg["env"]["config"], cfg["algo"]["config"]
+    cmd = algo["command"]
+    explore = list(env.get("exploration_params", ["mass"]))
+    G = len(explore) + 1
+    if engine is None:
+        sim = copy.deepcopy(SimParams())
+        sim.control_mode, sim.motor_model = 1, 3
+        engine = RolloutEngine(sim)
+    apply_y = bool(cfg.get("simulator", {}).get("config", {}).get("apply_inertiay", False))
+    theta = theta_from_default_param(env.get("default_param", {}), apply_inertiay=apply_y)
+    rows = _param_rows(apply_y)
+    names = [PARAM_NAMES[rows[p]] if p in rows else p for p in explore]
+    roll = PolicyFIMRollout(engine, policy, theta, names, delta=float(env.get("delta_param", 0.1)),
+                            ksync_steps=int(env.get("ksync_steps", 1)),
+                            termination_rew=float(cmd.get("termination_rew", 0.0)))
+    dt = engine.sim.dt * engine.sim.decimation
+    gen = CommandGenerator(cmd["command_ranges"], cmd["default_command"], cmd["command_sampling_idxs"],
+                           cmd.get("command_sampling_mode", "constant"), float(cmd["rollout_length"]),
+                           float(cmd["horizon_length"]), dt, int(cmd.get("poly_degree", 3)),
+                           int(cmd.get("num_bezier_points", 4)))
+    batch = int(cfg.get("num_envs", 2048)) // G
+    sampler = algo.get("optimize", {}).get("sampler", {})
+    if "batch_size" in sampler:
+        batch = int(sampler["batch_size"])
+    return PolicyActiveSysId(roll, gen, batch_size=batch, seed=seed)

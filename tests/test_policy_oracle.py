@@ -172,3 +172,24 @@ def test_zero_policy_rollout_is_open_loop_fim(model, sim, theta0):
                 for k in range(1, G):
                     states[k] = states[0].copy()
         assert fim[m] == pytest.approx(tot / (T - 1), rel=1e-4), m
+
+
+def test_default_param_rows_follow_the_reference_setters():
+    """env/active_sysid_openloop.yaml default_param: mass 9.39 on the base, CoM 0, inertia 0.005,
+    motor A 20; 'inertiay' stays unapplied (the reference's setter is set_inertiaiy) unless asked."""
+    import yaml
+    from spi_amd.agents.policy import theta_from_default_param
+    from spi_amd.params import P_COMX, P_IXX, P_IYY, P_IZZ, P_MASS, P_MOTOR0, nominal_params
+    here = os.path.dirname(os.path.abspath(__file__))
+    with open(os.path.join(here, "..", "spi-active_amd", "spi_amd", "configs", "env",
+                           "active_sysid_openloop.yaml")) as f:
+        env = yaml.safe_load(f)["env"]["config"]
+    nom = nominal_params()
+    th = theta_from_default_param(env["default_param"], nom)
+    assert th[P_MASS] == 9.39 and np.all(th[P_COMX:P_COMX + 3] == 0.0)
+    assert th[P_IXX] == 0.005 and th[P_IZZ] == 0.005 and th[P_IYY] == nom[P_IYY]
+    assert np.all(th[P_MOTOR0:P_MOTOR0 + 3] == 20.0)
+    assert theta_from_default_param(env["default_param"], nom, apply_inertiay=True)[P_IYY] == 0.005
+    with open(os.path.join(here, "..", "spi-active_amd", "spi_amd", "configs", "algo", "active_sysid.yaml")) as f:
+        algo = yaml.safe_load(f)["algo"]["config"]
+    assert algo["command"]["command_sampling_idxs"] == [0, 2, 5]
