@@ -51,6 +51,9 @@ def parse():
                    help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
                         "64 batched uneven problems, Fisher scoring of excitations and the serving loop")
     p.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
+    p.add_argument("--policy-batches", default="1024,8192",
+                   help="policy: trials per iteration at the reference batch[,wide batch]")
+    p.add_argument("--obs-layout", default="ring", choices=["ring", "dense"], help="policy: observation layout")
     p.add_argument("--profile-launch", action="store_true",
                    help="run one rollout_cost launch of the workload and a 256 MiB device copy (for rocprofv3 "
                         "--pmc FETCH_SIZE / WRITE_SIZE calibration), then exit")
@@ -71,6 +74,10 @@ def main():
         from spi_amd import workloads
         if args.workload == "serve":
             out = workloads.serve(n_requests=args.requests)
+        elif args.workload == "policy":
+            bs = [int(x) for x in args.policy_batches.split(",")]
+            out = workloads.policy(args.steps, args.warmup, batch=bs[0], wide_batch=bs[1] if len(bs) > 1 else 0,
+                                   obs_layout=args.obs_layout)
         else:
             out = getattr(workloads, args.workload)(args.steps, args.warmup)
         print(json.dumps(out))

@@ -376,6 +376,18 @@ int spi_policy_obs(int n_env, int group, const float* states_dev, const float* c
                    const float* applied_dev, float* history_dev, float* obs_dev, int evaluating, float height_lo,
                    float height_hi, float obs_clip, hipStream_t stream);
 
+/* Ring form of the same observation, without the history shift: this step's 60 terms go to
+ * slot `head` of ring_dev [spi_policy_ring_slots()][60][ld] (env index fastest; ld >= n_env);
+ * the actor's first layer gathers the 900-wide row from the ring (spi_mlp_forward_ring with the
+ * table of spi_policy_obs_kmap).  Slot head = step % spi_policy_ring_slots(), ring zeroed at reset. */
+int spi_policy_terms(int n_env, int group, const float* states_dev, const float* commands_dev, int n_cmd_steps,
+                     int step_now, int step_next, float dt_ctrl, const float* q_default_dev, float* gait_dev,
+                     const float* applied_dev, float* ring_dev, int ld, int head, int evaluating, float height_lo,
+                     float height_hi, hipStream_t stream);
+int spi_policy_ring_slots(void);
+/* Host table kmap_host[900][2] = (slots back from head, term index) of every observation column. */
+int spi_policy_obs_kmap(int* kmap_host);
+
 /* Full-state copy of each main env into its aux envs (a corrected sync; the reference's reaches
  * only the next first torque, see spi_policy_step). */
 int spi_policy_sync_state(float* states_dev, int n_env, int group, hipStream_t stream);
@@ -387,6 +399,12 @@ long long spi_mlp_param_floats(int n_layers, const int* dims);
 long long spi_mlp_work_floats(int M, int n_layers, const int* dims);
 int spi_mlp_forward(const float* x_dev, int M, int n_layers, const int* dims, const float* params_dev, float* work_dev,
                     long long work_floats, float* y_dev, hipStream_t stream);
+/* The same network with its input row m gathered from a ring (see spi_policy_terms):
+ * x[m][k] = clip(ring_dev[((head - kmap[k][0]) mod n_slots) * n_terms + kmap[k][1]][m], +-clip),
+ * kmap_dev [dims[0]][2] int (8-byte aligned); needs dims[0] % 4 == 0. */
+int spi_mlp_forward_ring(const float* ring_dev, int ld, int n_slots, int n_terms, int head, const int* kmap_dev,
+                         float clip, int M, int n_layers, const int* dims, const float* params_dev, float* work_dev,
+                         long long work_floats, float* y_dev, hipStream_t stream);
 
 #ifdef __cplusplus
 }

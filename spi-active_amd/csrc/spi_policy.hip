@@ -35,6 +35,8 @@ constexpr float kTwoPi = 6.28318530717958647692f;
 //  projected_gravity 3
 __constant__ int kOff[kKeys] = {0, 12, 15, 19, 20, 22, 23, 24, 25, 29, 31, 33, 45, 57};
 __constant__ int kDim[kKeys] = {12, 3, 4, 1, 2, 1, 1, 1, 4, 2, 2, 12, 12, 3};
+constexpr int kOffHost[kKeys] = {0, 12, 15, 19, 20, 22, 23, 24, 25, 29, 31, 33, 45, 57};
+constexpr int kDimHost[kKeys] = {12, 3, 4, 1, 2, 1, 1, 1, 4, 2, 2, 12, 12, 3};
 
 __device__ __forceinline__ LaneParams load_lane(const float* __restrict__ par, int ld, int i) {
   LaneParams lp;
@@ -147,25 +149,19 @@ __global__ void policy_reward_kernel(const float* __restrict__ s, int n_group, i
 }
 
 // Gait clock of go2_omni (go2_omni.py:348-379: gait index advanced by dt * frequency, foot
-// indices from phase / offset / bound, warped by the stance duration), observation terms
-// (legged_robot_base.py:795-805, 843-844; go2_locomotion.py:273-277; go2_omni.py:627-668 with the
-// evaluation-time body-height sinusoid), sorted concatenation + short history per key
-// (legged_robot_base.py:503-527, 819-829), clip (legged_base.yaml clip_observations), history shift
-// (history_handler.py:36-40).  cmd_now: the commands the step ran with (gait clock); cmd_next:
-// the commands after the playback pointer advanced (observation, active_sysid_openloop.py:196-201).
-// hist: [n][kHist][kTerms], newest first.
-__global__ __launch_bounds__(kPolThreads) void policy_obs_kernel(
-    int n, int group, const float* __restrict__ s, const float* __restrict__ cmds, int T, int s_now, int s_next,
-    float dt_ctrl, const float* __restrict__ q_default, float* __restrict__ gait, const float* __restrict__ applied,
-    float* __restrict__ hist, float* __restrict__ obs, int evaluating, float h_lo, float h_hi, float obs_clip) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int g = i / group;
-  const float* cn = cmds + (static_cast<size_t>(g) * T + s_now) * 14;
-  const float* cx = cmds + (static_cast<size_t>(g) * T + s_next) * 14;
-  float gi = gait[i] + dt_ctrl * cn[4];
+// indices from phase / offset / bound, warped by the stance duration) and the scaled
+// observation terms in sorted key order (legged_robot_base.py:795-805, 843-844;
+// go2_locomotion.py:273-277; go2_omni.py:627-668 with the evaluation-time body-height
+// sinusoid).  cn: the commands the step ran with (gait clock); cx: the commands after the
+// playback pointer advanced (observation, active_sysid_openloop.py:196-201).
+__device__ __forceinline__ void obs_terms(const float* __restrict__ x, const float* __restrict__ cn,
+                                          const float* __restrict__ cx, float dt_ctrl,
+                                          const float* __restrict__ q_default, float* __restrict__ gait_i,
+                                          const float* __restrict__ applied_i, int evaluating, float h_lo, float h_hi,
+                                          float* cur) {
+  float gi = *gait_i + dt_ctrl * cn[4];
   gi = gi - floorf(gi);
-  gait[i] = gi;
+  *gait_i = gi;
   const float feet[4] = {gi + cn[5] + cn[6] + cn[7], gi + cn[6], gi + cn[7], gi + cn[5]};
   const float dur = cn[8];
   float clock[4];
@@ -177,18 +173,16 @@ __global__ __launch_bounds__(kPolThreads) void policy_obs_kernel(
     else if (r > dur) idx = 0.5f + (r - dur) * (0.5f / (1.0f - dur));
     clock[f] = sinf(kTwoPi * idx);
   }
-  const float* x = s + static_cast<size_t>(i) * SPI_STATE_DIM;
   float ang[3], pg[3];
   quat_rotate_inverse(x[3], x[4], x[5], x[6], x[10], x[11], x[12], ang);
   quat_rotate_inverse(x[3], x[4], x[5], x[6], 0.0f, 0.0f, -1.0f, pg);
   float h = cx[3];
   if (evaluating) h = fminf(fmaxf(cx[3] + 0.20f * sinf(kTwoPi * gi), h_lo), h_hi);
-  float cur[kTerms];
 #pragma unroll
   for (int j = 0; j < 12; ++j) {
-    cur[j] = applied[static_cast<size_t>(i) * 12 + j];  // actions
-    cur[33 + j] = x[13 + j] - q_default[j];              // dof_pos
-    cur[45 + j] = 0.05f * x[25 + j];                     // dof_vel
+    cur[j] = applied_i[j];                    // actions
+    cur[33 + j] = x[13 + j] - q_default[j];   // dof_pos
+    cur[45 + j] = 0.05f * x[25 + j];          // dof_vel
   }
   cur[12] = 0.25f * ang[0]; cur[13] = 0.25f * ang[1]; cur[14] = 0.25f * ang[2];  // base_ang_vel
 #pragma unroll
@@ -202,6 +196,22 @@ __global__ __launch_bounds__(kPolThreads) void policy_obs_kernel(
   cur[29] = cx[0]; cur[30] = cx[1];                                     // command_lin_vel
   cur[31] = cx[12]; cur[32] = cx[13];                                   // command_stance
   cur[57] = pg[0]; cur[58] = pg[1]; cur[59] = pg[2];                    // projected_gravity
+}
+
+// Dense observation: sorted terms + short history per key (legged_robot_base.py:503-527,
+// 819-829), clipped (legged_base.yaml clip_observations), then the history shift
+// (history_handler.py:36-40).  hist: [n][kHist][kTerms], newest first; obs [n][kObs].
+__global__ __launch_bounds__(kPolThreads) void policy_obs_kernel(
+    int n, int group, const float* __restrict__ s, const float* __restrict__ cmds, int T, int s_now, int s_next,
+    float dt_ctrl, const float* __restrict__ q_default, float* __restrict__ gait, const float* __restrict__ applied,
+    float* __restrict__ hist, float* __restrict__ obs, int evaluating, float h_lo, float h_hi, float obs_clip) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = i / group;
+  float cur[kTerms];
+  obs_terms(s + static_cast<size_t>(i) * SPI_STATE_DIM, cmds + (static_cast<size_t>(g) * T + s_now) * 14,
+            cmds + (static_cast<size_t>(g) * T + s_next) * 14, dt_ctrl, q_default, gait + i,
+            applied + static_cast<size_t>(i) * 12, evaluating, h_lo, h_hi, cur);
   float* o = obs + static_cast<size_t>(i) * kObs;
 #pragma unroll
   for (int e = 0; e < kTerms; ++e) o[e] = fminf(fmaxf(cur[e], -obs_clip), obs_clip);
@@ -216,6 +226,26 @@ __global__ __launch_bounds__(kPolThreads) void policy_obs_kernel(
     for (int e = 0; e < kTerms; ++e) hb[t * kTerms + e] = hb[(t - 1) * kTerms + e];
 #pragma unroll
   for (int e = 0; e < kTerms; ++e) hb[e] = cur[e];
+}
+
+// Ring observation: this step's terms go to slot `head` of ring[kHist + 1][kTerms][ld] (env
+// index fastest, so the writes of a wavefront are coalesced).  The observation row is then
+// [slot head | slot head - 1 - t of each key, t < kHist], which the actor's first layer
+// gathers directly (spi_mlp_forward_ring with spi_policy_obs_kmap); nothing is shifted.
+__global__ __launch_bounds__(kPolThreads) void policy_terms_kernel(
+    int n, int group, const float* __restrict__ s, const float* __restrict__ cmds, int T, int s_now, int s_next,
+    float dt_ctrl, const float* __restrict__ q_default, float* __restrict__ gait, const float* __restrict__ applied,
+    float* __restrict__ ring, int ld, int head, int evaluating, float h_lo, float h_hi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = i / group;
+  float cur[kTerms];
+  obs_terms(s + static_cast<size_t>(i) * SPI_STATE_DIM, cmds + (static_cast<size_t>(g) * T + s_now) * 14,
+            cmds + (static_cast<size_t>(g) * T + s_next) * 14, dt_ctrl, q_default, gait + i,
+            applied + static_cast<size_t>(i) * 12, evaluating, h_lo, h_hi, cur);
+  float* r = ring + static_cast<size_t>(head) * kTerms * ld + i;
+#pragma unroll
+  for (int e = 0; e < kTerms; ++e) r[static_cast<size_t>(e) * ld] = cur[e];
 }
 
 // Full state copy main -> aux envs (the state-sync variant; the reference's sync only reaches the
@@ -290,6 +320,33 @@ int spi_policy_obs(int n_env, int group, const float* states_dev, const float* c
       n_env, group, states_dev, commands_dev, n_cmd_steps, step_now, step_next, dt_ctrl, q_default_dev, gait_dev,
       applied_dev, history_dev, obs_dev, evaluating, height_lo, height_hi, obs_clip);
   return plaunch("policy_obs_kernel");
+}
+
+int spi_policy_terms(int n_env, int group, const float* states_dev, const float* commands_dev, int n_cmd_steps,
+                     int step_now, int step_next, float dt_ctrl, const float* q_default_dev, float* gait_dev,
+                     const float* applied_dev, float* ring_dev, int ld, int head, int evaluating, float height_lo,
+                     float height_hi, hipStream_t stream) {
+  if (!groups_ok(n_env, group) || n_cmd_steps < 1 || step_now < 0 || step_next < 0 || step_now >= n_cmd_steps ||
+      step_next >= n_cmd_steps || !(dt_ctrl > 0.0f) || ld < n_env || head < 0 || head > kHist || !states_dev ||
+      !commands_dev || !q_default_dev || !gait_dev || !applied_dev || !ring_dev)
+    return pfail("spi_policy_terms: bad arguments");
+  if (n_env == 0) return SPI_OK;
+  policy_terms_kernel<<<(n_env + kPolThreads - 1) / kPolThreads, kPolThreads, 0, stream>>>(
+      n_env, group, states_dev, commands_dev, n_cmd_steps, step_now, step_next, dt_ctrl, q_default_dev, gait_dev,
+      applied_dev, ring_dev, ld, head, evaluating, height_lo, height_hi);
+  return plaunch("policy_terms_kernel");
+}
+
+int spi_policy_ring_slots(void) { return kHist + 1; }
+
+int spi_policy_obs_kmap(int* kmap_host) {
+  if (!kmap_host) return pfail("spi_policy_obs_kmap: null output");
+  int w = 0;
+  for (int e = 0; e < kTerms; ++e, ++w) { kmap_host[2 * w] = 0; kmap_host[2 * w + 1] = e; }
+  for (int k = 0; k < kKeys; ++k)
+    for (int t = 0; t < kHist; ++t)
+      for (int e = 0; e < kDimHost[k]; ++e, ++w) { kmap_host[2 * w] = 1 + t; kmap_host[2 * w + 1] = kOffHost[k] + e; }
+  return w == kObs ? SPI_OK : pfail("spi_policy_obs_kmap: layout mismatch");
 }
 
 int spi_policy_sync_state(float* states_dev, int n_env, int group, hipStream_t stream) {

@@ -128,10 +128,14 @@ def lib() -> C.CDLL:
         "spi_policy_obs": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, vp, ip, fp, fp, fp, vp],
         "spi_policy_sync_state": [vp, ip, ip, vp],
         "spi_mlp_forward": [vp, ip, ip, C.POINTER(ip), vp, vp, C.c_longlong, vp, vp],
+        "spi_mlp_forward_ring": [vp, ip, ip, ip, ip, vp, fp, ip, ip, C.POINTER(ip), vp, vp, C.c_longlong, vp, vp],
+        "spi_policy_terms": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, ip, ip, ip, fp, fp, vp],
+        "spi_policy_obs_kmap": [C.POINTER(ip)],
     }
     L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
     L.spi_adjoint_work_floats.restype = C.c_longlong
     L.spi_policy_obs_dim.restype = ip
+    L.spi_policy_ring_slots.restype = ip
     for name in ("spi_mlp_param_floats", "spi_mlp_work_floats"):
         getattr(L, name).restype = C.c_longlong
     L.spi_mlp_param_floats.argtypes = [ip, C.POINTER(ip)]

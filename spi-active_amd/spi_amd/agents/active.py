@@ -49,8 +49,9 @@ class FDFIMObjective:
             init = init[None].expand(M * G, 37).contiguous()
         par = self.params(M)
         out = torch.empty(M, dtype=torch.float32, device=self.engine.device)
+        actions = actions.contiguous()   # keep the (possibly copied) operand alive through the launch
         rc = _lib.lib().spi_fim_rollout(C_ref(self.engine.args), _lib.ptr(par), par.shape[1], M, self.P,
-                                        _lib.ptr(init), _lib.ptr(actions.contiguous()), T, self.delta, self.ksync,
+                                        _lib.ptr(init), _lib.ptr(actions), T, self.delta, self.ksync,
                                         self.ksync_mode, _lib.ptr(out), _lib.stream())
         _lib.check(rc, "spi_fim_rollout")
         return out / T if mean else out

@@ -88,8 +88,9 @@ class FisherObjective:
         J = torch.empty((G, H * FIM_ROWS, self.P), dtype=torch.float32, device=dev)
         st = torch.empty(G, dtype=torch.int32, device=dev)
         y = torch.empty((G, H, FIM_ROWS), dtype=torch.float32, device=dev) if with_y else None
+        actions = actions.contiguous()   # keep the (possibly copied) operand alive through the launch
         rc = _lib.lib().spi_traj_sensitivity(C_ref(self.engine.args), C_ref(self.spec), _lib.ptr(theta), S, S,
-                                             _lib.ptr(init), _lib.ptr(actions.contiguous()), E, H, _lib.ptr(J),
+                                             _lib.ptr(init), _lib.ptr(actions), E, H, _lib.ptr(J),
                                              _lib.ptr(y), _lib.ptr(st, torch.int32), _lib.stream())
         _lib.check(rc, "spi_traj_sensitivity")
         return J, st, y

@@ -99,18 +99,36 @@ class PolicyMLP:
             off += o
         return out
 
+    def _work_for(self, M: int) -> torch.Tensor:
+        need = _lib.lib().spi_mlp_work_floats(M, self.n_layers, self._dims_c)
+        if self._work is None or self._work.numel() < need:
+            self._work = torch.empty(max(need, 1), dtype=torch.float32, device=self.device)
+        return self._work
+
+    def forward_ring(self, ring: torch.Tensor, head: int, kmap: torch.Tensor, clip: float, M: int,
+                     out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Input rows gathered from a term ring [n_slots, n_terms, ld] (spi_mlp_forward_ring)."""
+        S, n_terms, ld = ring.shape
+        if kmap.shape != (self.dims[0], 2) or kmap.dtype != torch.int32:
+            raise ValueError(f"kmap must be int32 [{self.dims[0]}, 2]")
+        work = self._work_for(M)
+        if out is None:
+            out = torch.empty((M, self.dims[-1]), dtype=torch.float32, device=self.device)
+        rc = _lib.lib().spi_mlp_forward_ring(_lib.ptr(ring), ld, S, n_terms, int(head), _lib.ptr(kmap, torch.int32),
+                                             float(clip), M, self.n_layers, self._dims_c, _lib.ptr(self.params),
+                                             _lib.ptr(work), work.numel(), _lib.ptr(out), _lib.stream())
+        _lib.check(rc, "spi_mlp_forward_ring")
+        return out
+
     def __call__(self, x: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
         if x.dim() != 2 or x.shape[1] != self.dims[0]:
             raise ValueError(f"input must be [M, {self.dims[0]}]")
         M = x.shape[0]
-        L = _lib.lib()
-        need = L.spi_mlp_work_floats(M, self.n_layers, self._dims_c)
-        if self._work is None or self._work.numel() < need:
-            self._work = torch.empty(max(need, 1), dtype=torch.float32, device=self.device)
+        work = self._work_for(M)
         if out is None:
             out = torch.empty((M, self.dims[-1]), dtype=torch.float32, device=self.device)
-        rc = L.spi_mlp_forward(_lib.ptr(x), M, self.n_layers, self._dims_c, _lib.ptr(self.params),
-                               _lib.ptr(self._work), self._work.numel(), _lib.ptr(out), _lib.stream())
+        rc = _lib.lib().spi_mlp_forward(_lib.ptr(x), M, self.n_layers, self._dims_c, _lib.ptr(self.params),
+                                        _lib.ptr(work), work.numel(), _lib.ptr(out), _lib.stream())
         _lib.check(rc, "spi_mlp_forward")
         return out
 
@@ -136,7 +154,10 @@ class PolicyFIMRollout:
     def __init__(self, engine: RolloutEngine, policy: PolicyMLP, theta: np.ndarray, explore: Sequence[str],
                  delta: float = 0.1, ksync_steps: int = 5, ksync_mode: str = "reference",
                  termination_rew: float = 0.0, gravity_limits=(0.8, 0.8), height_limits=(-0.25, 0.15),
-                 obs_clip: float = 100.0, evaluating: bool = True):
+                 obs_clip: float = 100.0, evaluating: bool = True, obs_layout: str = "ring"):
+        if obs_layout not in ("ring", "dense"):
+            raise ValueError("obs_layout must be 'ring' or 'dense'")
+        self.obs_layout = obs_layout
         self.engine = engine
         self.policy = policy
         obs_dim = _lib.lib().spi_policy_obs_dim()
@@ -181,9 +202,18 @@ class PolicyFIMRollout:
         f = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)
         i = lambda *s: torch.zeros(s, dtype=torch.int32, device=dev)
         b = {"M": M, "T": T, "n": n, "par": self.params(M), "s": [f(n, 37), f(n, 37)], "init": f(n, 37),
-             "actions": f(n, 12), "applied": f(n, 12), "done": i(n), "bad": i(n), "gait": f(n),
-             "hist": f(n, OBS_HISTORY, OBS_TERMS), "obs": f(n, self.obs_dim), "acc": f(M), "gbad": i(M),
-             "cmds": f(M, T, 14)}
+             "actions": f(n, 12), "applied": f(n, 12), "done": i(n), "bad": i(n), "gait": f(n), "acc": f(M),
+             "gbad": i(M), "cmds": f(M, T, 14)}
+        if self.obs_layout == "ring":
+            L = _lib.lib()
+            S = L.spi_policy_ring_slots()
+            km = (C.c_int * (2 * self.obs_dim))()
+            _lib.check(L.spi_policy_obs_kmap(km), "spi_policy_obs_kmap")
+            b["ring"] = f(S, OBS_TERMS, n)
+            b["kmap"] = torch.tensor(list(km), dtype=torch.int32).view(self.obs_dim, 2).to(dev)
+        else:
+            b["hist"] = f(n, OBS_HISTORY, OBS_TERMS)
+            b["obs"] = f(n, self.obs_dim)
         self._bufs = b
         self._graph = None
         return b
@@ -197,8 +227,10 @@ class PolicyFIMRollout:
         par = b["par"]
         s = b["s"]
         s[0].copy_(b["init"])
-        for k in ("actions", "done", "bad", "gait", "hist", "acc", "gbad"):
+        ring = self.obs_layout == "ring"
+        for k in ("actions", "done", "bad", "gait", "acc", "gbad") + (("ring",) if ring else ("hist",)):
             b[k].zero_()
+        S = b["ring"].shape[0] if ring else 0
 
         def physics(cur: int, sync_first: bool) -> None:
             rc = L.spi_policy_step(a, _lib.ptr(par), par.shape[1], n, G, _lib.ptr(s[cur]), _lib.ptr(s[1 - cur]),
@@ -207,11 +239,24 @@ class PolicyFIMRollout:
             _lib.check(rc, "spi_policy_step")
 
         def observe(cur: int, now: int) -> None:
+            if ring:                     # step `now` writes ring slot now % S
+                rc = L.spi_policy_terms(n, G, _lib.ptr(s[cur]), _lib.ptr(b["cmds"]), T, now, now + 1, self.dt_ctrl,
+                                        _lib.ptr(self.q_default), _lib.ptr(b["gait"]), _lib.ptr(b["applied"]),
+                                        _lib.ptr(b["ring"]), n, now % S, self.evaluating, self.height[0],
+                                        self.height[1], st)
+                _lib.check(rc, "spi_policy_terms")
+                return
             rc = L.spi_policy_obs(n, G, _lib.ptr(s[cur]), _lib.ptr(b["cmds"]), T, now, now + 1, self.dt_ctrl,
                                   _lib.ptr(self.q_default), _lib.ptr(b["gait"]), _lib.ptr(b["applied"]),
                                   _lib.ptr(b["hist"]), _lib.ptr(b["obs"]), self.evaluating, self.height[0],
                                   self.height[1], self.obs_clip, st)
             _lib.check(rc, "spi_policy_obs")
+
+        def act(latest: int) -> None:
+            if ring:
+                self.policy.forward_ring(b["ring"], latest % S, b["kmap"], self.obs_clip, n, out=b["actions"])
+            else:
+                self.policy(b["obs"], out=b["actions"])
 
         def sync(cur: int, step_idx: int) -> bool:
             if not self.sync_after(step_idx):
@@ -228,7 +273,7 @@ class PolicyFIMRollout:
         observe(cur, 0)
         pending = sync(cur, 1)
         for it in range(1, T - 1):
-            self.policy(b["obs"], out=b["actions"])
+            act(it - 1)
             physics(cur, pending)
             cur = 1 - cur
             rc = L.spi_policy_reward(_lib.ptr(s[cur]), M, G, self.delta, self.grav[0], self.grav[1],

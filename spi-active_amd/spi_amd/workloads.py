@@ -323,12 +323,13 @@ def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, h
     # forward mode on a slice of the segments (22 lanes per rollout)
     spec = _lib.tangent_spec(_rows(IDENTIFIED))
     n_sl = max(1, n_seg // 16)
+    seg_sl = seg_dev[:n_sl].contiguous()
     c2 = torch.empty((n_sl, n_samples), dtype=torch.float32, device="cuda")
     g2 = torch.empty((n_sl, n_samples, spec.n_tan), dtype=torch.float32, device="cuda")
     a, b = _events(st)
     a.record(st)
     rc = _lib.lib().spi_rollout_cost_grad(C_ref(eng.args), C_ref(eng.cargs), C_ref(spec), _lib.ptr(theta),
-                                          n_samples, n_samples, _lib.ptr(seg_dev[:n_sl].contiguous()), n_sl, horizon,
+                                          n_samples, n_samples, _lib.ptr(seg_sl), n_sl, horizon,
                                           _lib.ptr(c2), _lib.ptr(g2), _lib.stream())
     _lib.check(rc, "spi_rollout_cost_grad")
     b.record(st)
@@ -347,7 +348,8 @@ def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, h
             "forward_mode_over_adjoint": round(tan_ms_full / adj_ms, 2)}
 
 
-def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, total_steps: int = 1250) -> Dict:
+def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, total_steps: int = 1250,
+           obs_layout: str = "ring") -> Dict:
     """Policy-in-the-loop active exploration (run_active_sysid.py end to end on the device): each
     iteration samples ``batch`` command trials (constant mode, dims 0, 2, 5, 5 windows), expands
     them into 1250-step command sequences, rolls out (1 main + 1 aux) env groups driven by the
@@ -370,8 +372,9 @@ def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, t
     net = PolicyMLP.random(900, 12, seed=0)
     st = torch.cuda.current_stream()
     res = {}
-    for name, B in (("reference_batch", batch), ("wide_batch", wide_batch)):
-        roll = PolicyFIMRollout(eng, net, theta, ["mass"], delta=0.1, ksync_steps=5)
+    runs = [("reference_batch", batch)] + ([("wide_batch", wide_batch)] if wide_batch else [])
+    for name, B in runs:
+        roll = PolicyFIMRollout(eng, net, theta, ["mass"], delta=0.1, ksync_steps=5, obs_layout=obs_layout)
         drv = PolicyActiveSysId(roll, gen, batch_size=B, seed=1)
         for _ in range(warmup):
             drv.step()
@@ -413,4 +416,4 @@ def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, t
             "ms_per_step": ref["ms_per_iteration"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "fp32", "data": "synthetic (random actor weights)",
             "config": {"workload": "policy_active_sysid", "explore": ["mass"], "total_steps": total_steps,
-                       "command_mode": "constant", "ksync_steps": 5}, **res}
+                       "command_mode": "constant", "ksync_steps": 5, "obs_layout": obs_layout}, **res}

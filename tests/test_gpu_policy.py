@@ -71,12 +71,14 @@ def test_obs_kernel_matches_oracle(gpu, sim):
     obs = torch.zeros((n, 900), device="cuda")
     g_ref = np.zeros(n)
     h_ref = [op.empty_history() for _ in range(n)]
+    cmds_d, q0_d = _dev(cmds), _dev(q0)               # device buffers stay referenced while in use
     for step in range(3):
         states = np.stack([random_state(rng, sim) for _ in range(n)]).astype(np.float32)
         states[1, 25:37] = 5000.0                       # dof_vel obs 250 -> clipped
         applied = rng.uniform(-3, 3, (n, 12)).astype(np.float32)
-        rc = L.spi_policy_obs(n, G, _lib.ptr(_dev(states)), _lib.ptr(_dev(cmds)), T, step, step + 1,
-                              sim.dt * sim.decimation, _lib.ptr(_dev(q0)), _lib.ptr(gait), _lib.ptr(_dev(applied)),
+        states_d, applied_d = _dev(states), _dev(applied)
+        rc = L.spi_policy_obs(n, G, _lib.ptr(states_d), _lib.ptr(cmds_d), T, step, step + 1,
+                              sim.dt * sim.decimation, _lib.ptr(q0_d), _lib.ptr(gait), _lib.ptr(applied_d),
                               _lib.ptr(hist), _lib.ptr(obs), 1, -0.25, 0.15, 100.0, _lib.stream())
         _lib.check(rc, "spi_policy_obs")
         got = obs.cpu().numpy()
@@ -108,9 +110,9 @@ def test_policy_step_matches_oracle(gpu, model, sim, theta0):
     done = np.array([0, 0, 1, 1], np.int32)
     out, applied, bad = torch.zeros((n, 37), device="cuda"), torch.zeros((n, 12), device="cuda"), _dev(
         np.zeros(n), torch.int32)
-    P = _dev(par.T)
-    rc = L.spi_policy_step(C.byref(eng.args), _lib.ptr(P), n, n, G, _lib.ptr(_dev(states)), _lib.ptr(out),
-                           _lib.ptr(_dev(actions)), _lib.ptr(_dev(done, torch.int32), torch.int32), 1,
+    P, states_d, actions_d, done_d = _dev(par.T), _dev(states), _dev(actions), _dev(done, torch.int32)
+    rc = L.spi_policy_step(C.byref(eng.args), _lib.ptr(P), n, n, G, _lib.ptr(states_d), _lib.ptr(out),
+                           _lib.ptr(actions_d), _lib.ptr(done_d, torch.int32), 1,
                            _lib.ptr(applied), _lib.ptr(bad, torch.int32), _lib.stream())
     _lib.check(rc, "spi_policy_step")
     got, app = out.cpu().numpy(), applied.cpu().numpy()
@@ -146,8 +148,8 @@ def test_reward_kernel_termination_and_fim(gpu):
     ang = np.pi / 3
     s[4, 3], s[4, 6] = np.sin(ang / 2), np.cos(ang / 2)
     done, bad = _dev(np.zeros(M * G), torch.int32), _dev([0] * 8 + [1], torch.int32)
-    acc, gbad = _dev(np.full(M, 0.25)), _dev(np.zeros(M), torch.int32)
-    rc = L.spi_policy_reward(_lib.ptr(_dev(s)), M, G, delta, 0.8, 0.8, trew, _lib.ptr(done, torch.int32),
+    acc, gbad, s_d = _dev(np.full(M, 0.25)), _dev(np.zeros(M), torch.int32), _dev(s)
+    rc = L.spi_policy_reward(_lib.ptr(s_d), M, G, delta, 0.8, 0.8, trew, _lib.ptr(done, torch.int32),
                              _lib.ptr(bad, torch.int32), _lib.ptr(acc), _lib.ptr(gbad, torch.int32), _lib.stream())
     _lib.check(rc, "spi_policy_reward")
     assert done.cpu().numpy().tolist() == [0, 0, 0, 1, 1, 1, 0, 0, 0]
@@ -243,3 +245,24 @@ def test_graph_replay_matches_eager(gpu, model, sim, theta0):
             assert torch.equal(torch.nan_to_num(g, nan=-7.0), torch.nan_to_num(eager, nan=-7.0)), rep
         outs.append(eager)
     assert not torch.equal(outs[0], outs[1])
+
+
+def test_ring_observation_equals_dense(gpu, model, sim, theta0):
+    """The ring layout (terms written once, the actor's first layer gathers the 900-wide row) and
+    the dense layout (materialised observation + shifted history) give bit-identical rollouts:
+    the gathered operands are the same floats in the same K order."""
+    from spi_amd.agents.policy import PolicyFIMRollout, PolicyMLP, reset_states
+    from spi_amd.rollout import RolloutEngine
+    s = _policy_sim(sim)
+    eng = RolloutEngine(s)
+    rng = np.random.default_rng(21)
+    M, T = 40, 40                                   # 40 steps: the 15-slot ring wraps twice
+    net = PolicyMLP.random(900, 12, seed=4)
+    init = _dev(reset_states(M * 2, s.q0, seed=3))
+    cmds = _dev(_commands(rng, M, T))
+    out = {}
+    for layout in ("ring", "dense"):
+        roll = PolicyFIMRollout(eng, net, theta0, ["mass"], obs_layout=layout)
+        out[layout] = (roll.evaluate(cmds, init).clone(), roll.final_states().clone())
+    assert torch.equal(torch.nan_to_num(out["ring"][0], nan=-7.0), torch.nan_to_num(out["dense"][0], nan=-7.0))
+    assert torch.equal(out["ring"][1], out["dense"][1])
