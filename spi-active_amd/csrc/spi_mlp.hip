@@ -1,9 +1,11 @@
 // Policy network inference (gfx950): the actor MLP of the locomotion policy (Linear layers with
 // ELU between them, spigym/agents/modules/modules.py:47-63; hidden sizes 512-256-128 from
-// config/algo/ppo.yaml) as FP32 MFMA (v_mfma_f32_32x32x2f32) tiles.  One launch per layer:
-// a workgroup of 4 wavefronts owns a 32-row x 128-column output tile; the row block of X and
-// the 128 weight rows are staged through LDS in K chunks of 64, padded to 65 words per row so
-// the column-wise operand reads hit distinct banks.  Strict IEEE float32 (no fast-math).
+// config/algo/ppo.yaml) as FP32 MFMA (v_mfma_f32_32x32x2f32) tiles, one launch per layer with
+// bias and ELU in the epilogue.  linear_kernel: a workgroup of 4 wavefronts owns a 32-row x
+// 128-column output tile (many workgroups for small batches); linear_big_kernel: 128 x 128
+// (a 64 x 64 quadrant per wavefront) for batches of 8192 rows and more.  Rows of X and the
+// weight rows are staged through LDS in K chunks of 64, padded to 65 words per row so the
+// column-wise operand reads hit distinct banks.  Strict IEEE float32 (no fast-math).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,11 +18,12 @@ namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
-constexpr int kRows = 32;    // output rows per workgroup (MFMA M)
 constexpr int kCols = 128;   // output columns per workgroup (4 wavefronts x MFMA N)
 constexpr int kKc = 64;      // K chunk staged per pass
 constexpr int kPad = kKc + 1;
 constexpr int kMaxLayers = 8;
+constexpr int kMaxRingK = 1024;  // widest ring-gathered input
+constexpr int kBigM = 8192;      // batches from here on use linear_big_kernel (N >= 256)
 
 // Layer input gathered from a ring of per-step observation terms, stored term-major
 // ring[slot][term][m] (m = row, fastest): X[m][k] = clip(ring[slot_k][term_k][m]) with
@@ -39,44 +42,69 @@ struct RingSrc {
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  The next K chunk is fetched from global memory
 // into registers while the MFMAs consume the current one from LDS.
 // kSrc 0: X row-major, scalar loads; 1: X and W rows 16-byte aligned with K % 4 == 0, float4
-// loads; 2: X gathered from a RingSrc (32 consecutive rows per load), W as in 1.
-template <bool kElu, int kSrc>
+// loads; 2: X gathered from a RingSrc (32 kRB consecutive rows of one column per load), W as in 1.
+template <bool kElu, int kSrc, int kRB>
 __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M, int K,
                                                      const float* __restrict__ W, const float* __restrict__ b, int N,
                                                      float* __restrict__ Y, int ldy) {
+  constexpr int kRows = 32 * kRB;
   __shared__ float As[kRows * kPad];
   __shared__ float Ws[kCols * kPad];
+  __shared__ int Kr[kSrc >= 2 ? kMaxRingK : 1];       // ring line of each input column
   const int m0 = blockIdx.x * kRows, n0 = blockIdx.y * kCols;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 31, half = lane >> 5;
   const bool live = n0 + wave * 32 < N;  // uniform per wavefront
-  floatx16 acc;
+  if (kSrc >= 2) {
+    // any table entry is folded into the ring: slots back in [0, n_slots), term in [0, n_terms)
+    for (int k = tid; k < K; k += 256) {
+      const int2 km = rs.kmap[k];
+      const int back = min(max(km.x, 0), rs.n_slots - 1);
+      const int slot = rs.head >= back ? rs.head - back : rs.head - back + rs.n_slots;
+      Kr[k] = slot * rs.n_terms + min(max(km.y, 0), rs.n_terms - 1);
+    }
+    __syncthreads();
+  }
+  floatx16 acc[kRB];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int q = 0; q < kRB; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
   constexpr int kV = kSrc == 0 ? 1 : 4;                // floats per W (and row-major X) load
   constexpr int kTPR = kKc / kV;                       // threads per tile row
   constexpr int kRowStep = 256 / kTPR;                 // rows covered by one pass of the block
   constexpr int kAV = kSrc == 2 ? 1 : kV;
-  constexpr int kAPer = kSrc == 2 ? kRows * kKc / 256 : kRows / kRowStep, kWPer = kCols / kRowStep;
+  // ring gather: thread -> row (tid % kRows), columns kk_r + i * (256 / kRows); float4 ring
+  // gather (kSrc 3): thread -> rows 4 (tid % (kRows / 4)) .. + 3, columns kk4 + i * (1024 / kRows)
+  constexpr int kGStep = 256 / kRows;
+  constexpr int kG4 = kRows / 4, kG4Step = 256 / kG4;
+  constexpr int kAPer = kSrc == 2 ? kRows * kKc / 256 : (kSrc == 3 ? kRows * kKc / 1024 : kRows / kRowStep);
+  constexpr int kWPer = kCols / kRowStep;
   const int r0 = tid / kTPR, c0 = (tid % kTPR) * kV;
-  const int gm_r = m0 + (tid & 31), kk_r = tid >> 5;  // ring gather: row, first K offset
+  const int gm_r = m0 + tid % kRows, kk_r = tid / kRows;
+  const int m4 = (tid % kG4) * 4, kk4 = tid / kG4;
   float ra[kAPer][kAV], rw[kWPer][kV];
   auto fetch = [&](int k0) {
     const int gk = k0 + c0;
 #pragma unroll
     for (int i = 0; i < kAPer; ++i) {
       if (kSrc == 2) {
-        const int k = k0 + kk_r + 8 * i;
+        const int k = k0 + kk_r + kGStep * i;
         float v = 0.0f;
         if (gm_r < M && k < K) {
-          const int2 km = rs.kmap[k];
-          int slot = (rs.head - km.x) % rs.n_slots;           // any table entry stays inside the ring
-          slot += slot < 0 ? rs.n_slots : 0;
-          const int term = min(max(km.y, 0), rs.n_terms - 1);
-          v = rs.ring[(static_cast<size_t>(slot) * rs.n_terms + term) * rs.ld + gm_r];
+          v = rs.ring[static_cast<size_t>(Kr[k]) * rs.ld + gm_r];
           v = fminf(fmaxf(v, -rs.clip), rs.clip);
         }
         ra[i][0] = v;
+      } else if (kSrc == 3) {
+        const int k = k0 + kk4 + kG4Step * i;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (m0 + m4 < M && k < K)
+          v = *reinterpret_cast<const float4*>(rs.ring + static_cast<size_t>(Kr[k]) * rs.ld + m0 + m4);
+        ra[i][0] = fminf(fmaxf(v.x, -rs.clip), rs.clip);
+        ra[i][kAV > 1 ? 1 : 0] = fminf(fmaxf(v.y, -rs.clip), rs.clip);
+        ra[i][kAV > 2 ? 2 : 0] = fminf(fmaxf(v.z, -rs.clip), rs.clip);
+        ra[i][kAV > 3 ? 3 : 0] = fminf(fmaxf(v.w, -rs.clip), rs.clip);
       } else {
         const int gm = m0 + r0 + i * kRowStep;
         if (kSrc == 1) {
@@ -105,7 +133,10 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
 #pragma unroll
     for (int i = 0; i < kAPer; ++i) {
       if (kSrc == 2) {
-        As[(tid & 31) * kPad + kk_r + 8 * i] = ra[i][0];
+        As[(tid % kRows) * kPad + kk_r + kGStep * i] = ra[i][0];
+      } else if (kSrc == 3) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) As[(m4 + v) * kPad + kk4 + kG4Step * i] = ra[i][v];
       } else {
 #pragma unroll
         for (int v = 0; v < kAV; ++v) As[(r0 + i * kRowStep) * kPad + c0 + v] = ra[i][v];
@@ -121,7 +152,12 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
       const float* ap = As + col * kPad + half;
       const float* bp = Ws + (wave * 32 + col) * kPad + half;
 #pragma unroll
-      for (int kk = 0; kk < kKc; kk += 2) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[kk], bp[kk], acc, 0, 0, 0);
+      for (int kk = 0; kk < kKc; kk += 2) {
+        const float bv = bp[kk];
+#pragma unroll
+        for (int q = 0; q < kRB; ++q)
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[q * 32 * kPad + kk], bv, acc[q], 0, 0, 0);
+      }
     }
     __syncthreads();
   }
@@ -130,13 +166,156 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
   if (gn >= N) return;
   const float bias = b[gn];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int gm = m0 + 8 * (r >> 2) + 4 * half + (r & 3);
-    if (gm < M) {
-      float v = acc[r] + bias;
-      if (kElu) v = v > 0.0f ? v : expm1f(v);
-      Y[static_cast<size_t>(gm) * ldy + gn] = v;
+  for (int q = 0; q < kRB; ++q)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int gm = m0 + 32 * q + 8 * (r >> 2) + 4 * half + (r & 3);
+      if (gm < M) {
+        float v = acc[q][r] + bias;
+        if (kElu) v = v > 0.0f ? v : expm1f(v);
+        Y[static_cast<size_t>(gm) * ldy + gn] = v;
+      }
     }
+}
+
+// Large batches: a workgroup of 4 wavefronts owns a 128 x 128 output tile, each wavefront a
+// 64 x 64 quadrant (2 x 2 MFMA blocks, 4 accumulators), so every LDS operand read feeds two
+// MFMAs and every staged byte 2 MFLOP / 64 KB of work.  K chunks of 64 as in linear_kernel,
+// the next chunk prefetched into registers.  kSrc 1: X row-major (rows 16-byte aligned,
+// K % 4 == 0); 2: X gathered from a RingSrc (64 consecutive rows of one column per load).
+constexpr int kBigRows = 128, kBigCols = 128;
+template <bool kElu, int kSrc>
+__global__ __launch_bounds__(256) void linear_big_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M,
+                                                         int K, const float* __restrict__ W,
+                                                         const float* __restrict__ b, int N, float* __restrict__ Y,
+                                                         int ldy) {
+  __shared__ float As[kBigRows * kPad];
+  __shared__ float Ws[kBigCols * kPad];
+  __shared__ int Kr[kSrc >= 2 ? kMaxRingK : 1];
+  const int m0 = blockIdx.x * kBigRows, n0 = blockIdx.y * kBigCols;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int col = lane & 31, half = lane >> 5;
+  const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;   // this wavefront's quadrant
+  const bool live = n0 + wc < N;                            // uniform per wavefront
+  if (kSrc >= 2) {
+    for (int k = tid; k < K; k += 256) {
+      const int2 km = rs.kmap[k];
+      const int back = min(max(km.x, 0), rs.n_slots - 1);
+      const int slot = rs.head >= back ? rs.head - back : rs.head - back + rs.n_slots;
+      Kr[k] = slot * rs.n_terms + min(max(km.y, 0), rs.n_terms - 1);
+    }
+    __syncthreads();
+  }
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[p][q][r] = 0.0f;
+  // row-major float4 staging: 16 threads per 64-float row, 16 rows per pass
+  const int r0 = tid >> 4, c0 = (tid & 15) * 4;
+  constexpr int kPer = 8;                                  // float4 per thread per operand
+  // ring gather: thread -> row tid % 128, columns tid / 128 + 2 i
+  const int gm_r = m0 + (tid & 127), kk_r = tid >> 7;
+  constexpr int kGPer = kBigRows * kKc / 256;              // 32
+  // float4 ring gather: thread -> rows 4 (tid % 32) .. + 3, columns tid / 32 + 8 i
+  const int m4 = (tid & 31) * 4, kk4 = tid >> 5;
+  float ra[kSrc == 2 ? kGPer : 4 * kPer];
+  float rw[4 * kPer];
+  auto fetch = [&](int k0) {
+    const int gk = k0 + c0;
+    if (kSrc == 2) {
+#pragma unroll
+      for (int i = 0; i < kGPer; ++i) {
+        const int k = k0 + kk_r + 2 * i;
+        float v = 0.0f;
+        if (gm_r < M && k < K) v = fminf(fmaxf(rs.ring[static_cast<size_t>(Kr[k]) * rs.ld + gm_r], -rs.clip), rs.clip);
+        ra[i] = v;
+      }
+    } else if (kSrc == 3) {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const int k = k0 + kk4 + 8 * i;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (m0 + m4 < M && k < K)
+          v = *reinterpret_cast<const float4*>(rs.ring + static_cast<size_t>(Kr[k]) * rs.ld + m0 + m4);
+        ra[4 * i] = fminf(fmaxf(v.x, -rs.clip), rs.clip);
+        ra[4 * i + 1] = fminf(fmaxf(v.y, -rs.clip), rs.clip);
+        ra[4 * i + 2] = fminf(fmaxf(v.z, -rs.clip), rs.clip);
+        ra[4 * i + 3] = fminf(fmaxf(v.w, -rs.clip), rs.clip);
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const int gm = m0 + r0 + 16 * i;
+        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (gm < M && gk < K) v = *reinterpret_cast<const float4*>(X + static_cast<size_t>(gm) * ldx + gk);
+        ra[4 * i] = v.x; ra[4 * i + 1] = v.y; ra[4 * i + 2] = v.z; ra[4 * i + 3] = v.w;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+      const int gn = n0 + r0 + 16 * i;
+      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (gn < N && gk < K) v = *reinterpret_cast<const float4*>(W + static_cast<size_t>(gn) * K + gk);
+      rw[4 * i] = v.x; rw[4 * i + 1] = v.y; rw[4 * i + 2] = v.z; rw[4 * i + 3] = v.w;
+    }
+  };
+  fetch(0);
+  for (int k0 = 0; k0 < K; k0 += kKc) {
+    if (kSrc == 2) {
+#pragma unroll
+      for (int i = 0; i < kGPer; ++i) As[(tid & 127) * kPad + kk_r + 2 * i] = ra[i];
+    } else if (kSrc == 3) {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) As[(m4 + v) * kPad + kk4 + 8 * i] = ra[4 * i + v];
+    } else {
+#pragma unroll
+      for (int i = 0; i < kPer; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) As[(r0 + 16 * i) * kPad + c0 + v] = ra[4 * i + v];
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Ws[(r0 + 16 * i) * kPad + c0 + v] = rw[4 * i + v];
+    __syncthreads();
+    if (k0 + kKc < K) fetch(k0 + kKc);
+    if (live) {
+      const float* ap = As + (wr + col) * kPad + half;
+      const float* bp = Ws + (wc + col) * kPad + half;
+#pragma unroll
+      for (int kk = 0; kk < kKc; kk += 2) {
+        const float a0 = ap[kk], a1 = ap[32 * kPad + kk];
+        const float b0 = bp[kk], b1 = bp[32 * kPad + kk];
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  if (!live) return;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int gn = n0 + wc + 32 * q + col;
+    if (gn >= N) continue;
+    const float bias = b[gn];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int gm = m0 + wr + 32 * p + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (gm < M) {
+          float v = acc[p][q][r] + bias;
+          if (kElu) v = v > 0.0f ? v : expm1f(v);
+          Y[static_cast<size_t>(gm) * ldy + gn] = v;
+        }
+      }
   }
 }
 
@@ -150,18 +329,37 @@ bool aligned16(const void* p) { return reinterpret_cast<uintptr_t>(p) % 16 == 0;
 // rs.ring != nullptr: X comes from the ring (K % 4 == 0 and W 16-byte aligned required).
 int launch_linear(const float* X, int ldx, const RingSrc& rs, int M, int K, const float* W, const float* b, int N,
                   float* Y, int ldy, bool elu, hipStream_t stream) {
-  const dim3 grid((M + kRows - 1) / kRows, (N + kCols - 1) / kCols);
   const bool wvec = K % 4 == 0 && aligned16(W);
-#define SPI_LINEAR(E, S) linear_kernel<E, S><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
-  if (rs.ring) {
-    if (!wvec) return mfail(SPI_ERR_INVALID_ARG, "ring-gathered layer needs K % 4 == 0 and 16-byte aligned weights");
-    if (elu) SPI_LINEAR(true, 2); else SPI_LINEAR(false, 2);
-  } else if (wvec && ldx % 4 == 0 && aligned16(X)) {
-    if (elu) SPI_LINEAR(true, 1); else SPI_LINEAR(false, 1);
+  const bool xvec = rs.ring ? true : (wvec && ldx % 4 == 0 && aligned16(X));
+  if (rs.ring && !wvec)
+    return mfail(SPI_ERR_INVALID_ARG, "ring-gathered layer needs K % 4 == 0 and 16-byte aligned weights");
+  // rows of the ring read as float4: needs ld % 4 == 0 (then rows up to round4(M) <= ld exist)
+  const bool ring4 = rs.ring && rs.ld % 4 == 0 && aligned16(rs.ring);
+  if (M >= kBigM && N >= 256 && wvec && xvec) {
+    const dim3 grid((M + kBigRows - 1) / kBigRows, (N + kBigCols - 1) / kBigCols);
+#define SPI_BIG(E, S) linear_big_kernel<E, S><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
+    if (ring4) {
+      if (elu) SPI_BIG(true, 3); else SPI_BIG(false, 3);
+    } else if (rs.ring) {
+      if (elu) SPI_BIG(true, 2); else SPI_BIG(false, 2);
+    } else {
+      if (elu) SPI_BIG(true, 1); else SPI_BIG(false, 1);
+    }
+#undef SPI_BIG
   } else {
-    if (elu) SPI_LINEAR(true, 0); else SPI_LINEAR(false, 0);
-  }
+    const dim3 grid((M + 31) / 32, (N + kCols - 1) / kCols);
+#define SPI_LINEAR(E, S) linear_kernel<E, S, 1><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
+    if (ring4) {
+      if (elu) SPI_LINEAR(true, 3); else SPI_LINEAR(false, 3);
+    } else if (rs.ring) {
+      if (elu) SPI_LINEAR(true, 2); else SPI_LINEAR(false, 2);
+    } else if (xvec) {
+      if (elu) SPI_LINEAR(true, 1); else SPI_LINEAR(false, 1);
+    } else {
+      if (elu) SPI_LINEAR(true, 0); else SPI_LINEAR(false, 0);
+    }
 #undef SPI_LINEAR
+  }
   const hipError_t e = hipGetLastError();
   if (e == hipSuccess) return SPI_OK;
   char buf[256];
@@ -234,8 +432,8 @@ int spi_mlp_forward_ring(const float* ring_dev, int ld, int n_slots, int n_terms
                          float clip, int M, int n_layers, const int* dims, const float* params_dev, float* work_dev,
                          long long work_floats, float* y_dev, hipStream_t stream) {
   if (!ring_dev || !kmap_dev || ld < M || n_slots < 1 || n_terms < 1 || head < 0 || head >= n_slots ||
-      !(clip > 0.0f) || reinterpret_cast<uintptr_t>(kmap_dev) % 8 != 0)
-    return mfail(SPI_ERR_INVALID_ARG, "spi_mlp_forward_ring: bad ring arguments");
+      !(clip > 0.0f) || reinterpret_cast<uintptr_t>(kmap_dev) % 8 != 0 || !dims || dims[0] > kMaxRingK)
+    return mfail(SPI_ERR_INVALID_ARG, "spi_mlp_forward_ring: bad ring arguments (input width <= 1024)");
   const RingSrc rs{ring_dev, ld, n_slots, n_terms, head, reinterpret_cast<const int2*>(kmap_dev), clip};
   return mlp_layers(nullptr, rs, M, n_layers, dims, params_dev, work_dev, work_floats, y_dev, stream);
 }

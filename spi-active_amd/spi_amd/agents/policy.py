@@ -21,6 +21,7 @@ base_task.py:90-100 and active_sysid.py:539-576):
 from __future__ import annotations
 
 import ctypes as C
+import math
 from typing import Dict, Optional, Sequence
 
 import numpy as np
@@ -105,18 +106,29 @@ class PolicyMLP:
             self._work = torch.empty(max(need, 1), dtype=torch.float32, device=self.device)
         return self._work
 
+    def work_floats(self, M: int) -> int:
+        return int(_lib.lib().spi_mlp_work_floats(M, self.n_layers, self._dims_c))
+
     def forward_ring(self, ring: torch.Tensor, head: int, kmap: torch.Tensor, clip: float, M: int,
-                     out: Optional[torch.Tensor] = None) -> torch.Tensor:
-        """Input rows gathered from a term ring [n_slots, n_terms, ld] (spi_mlp_forward_ring)."""
+                     out: Optional[torch.Tensor] = None, row0: int = 0,
+                     work: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Input rows row0 .. row0 + M - 1 gathered from a term ring [n_slots, n_terms, ld]
+        (spi_mlp_forward_ring); work: a private scratch buffer (for concurrent calls)."""
         S, n_terms, ld = ring.shape
         if kmap.shape != (self.dims[0], 2) or kmap.dtype != torch.int32:
             raise ValueError(f"kmap must be int32 [{self.dims[0]}, 2]")
-        work = self._work_for(M)
+        if not (ring.is_cuda and ring.is_contiguous() and ring.dtype == torch.float32) or row0 + M > ld:
+            raise ValueError("ring must be a contiguous float32 GPU tensor covering the rows")
+        if work is None:
+            work = self._work_for(M)
+        if work.numel() < self.work_floats(M):
+            raise ValueError("work buffer too small")
         if out is None:
             out = torch.empty((M, self.dims[-1]), dtype=torch.float32, device=self.device)
-        rc = _lib.lib().spi_mlp_forward_ring(_lib.ptr(ring), ld, S, n_terms, int(head), _lib.ptr(kmap, torch.int32),
-                                             float(clip), M, self.n_layers, self._dims_c, _lib.ptr(self.params),
-                                             _lib.ptr(work), work.numel(), _lib.ptr(out), _lib.stream())
+        rc = _lib.lib().spi_mlp_forward_ring(ring.data_ptr() + 4 * row0, ld, S, n_terms, int(head),
+                                             _lib.ptr(kmap, torch.int32), float(clip), M, self.n_layers,
+                                             self._dims_c, _lib.ptr(self.params), _lib.ptr(work), work.numel(),
+                                             _lib.ptr(out), _lib.stream())
         _lib.check(rc, "spi_mlp_forward_ring")
         return out
 
@@ -154,10 +166,14 @@ class PolicyFIMRollout:
     def __init__(self, engine: RolloutEngine, policy: PolicyMLP, theta: np.ndarray, explore: Sequence[str],
                  delta: float = 0.1, ksync_steps: int = 5, ksync_mode: str = "reference",
                  termination_rew: float = 0.0, gravity_limits=(0.8, 0.8), height_limits=(-0.25, 0.15),
-                 obs_clip: float = 100.0, evaluating: bool = True, obs_layout: str = "ring"):
+                 obs_clip: float = 100.0, evaluating: bool = True, obs_layout: str = "ring", n_streams: int = 1):
         if obs_layout not in ("ring", "dense"):
             raise ValueError("obs_layout must be 'ring' or 'dense'")
         self.obs_layout = obs_layout
+        # independent slices of the groups on their own streams: one slice's physics (latency-
+        # bound, a few wavefronts per SIMD) overlaps another's actor GEMMs (matrix cores)
+        self.n_streams = max(1, int(n_streams)) if obs_layout == "ring" else 1
+        self._streams = [torch.cuda.Stream() for _ in range(self.n_streams - 1)]
         self.engine = engine
         self.policy = policy
         obs_dim = _lib.lib().spi_policy_obs_dim()
@@ -209,80 +225,122 @@ class PolicyFIMRollout:
             S = L.spi_policy_ring_slots()
             km = (C.c_int * (2 * self.obs_dim))()
             _lib.check(L.spi_policy_obs_kmap(km), "spi_policy_obs_kmap")
-            b["ring"] = f(S, OBS_TERMS, n)
+            b["ring"] = f(S, OBS_TERMS, (n + 3) // 4 * 4)   # rows padded to 4: float4 gathers
             b["kmap"] = torch.tensor(list(km), dtype=torch.int32).view(self.obs_dim, 2).to(dev)
         else:
             b["hist"] = f(n, OBS_HISTORY, OBS_TERMS)
             b["obs"] = f(n, self.obs_dim)
+        b["parts"] = self._parts(M)
+        b["work"] = [f(max(self.policy.work_floats((g1 - g0) * self.G), 1)) for g0, g1 in b["parts"]]
         self._bufs = b
         self._graph = None
         return b
 
+    def _parts(self, M: int):
+        """Group ranges of the stream slices; slice starts are multiples of 4 envs when possible
+        (16-byte aligned ring rows)."""
+        G, P = self.G, self.n_streams
+        unit = 4 // math.gcd(G, 4)
+        cuts = sorted({0, M} | {min(M, int(round(k * M / P / unit)) * unit) for k in range(1, P)})
+        return [(int(a), int(b)) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+
     # ------------------------------------------------------------------ the rollout
     def _run(self, b: dict) -> None:
-        """Every launch of one rollout from b['init'] and b['cmds'] (capturable)."""
-        L, st = _lib.lib(), _lib.stream()
+        """Every launch of one rollout from b['init'] and b['cmds'] (capturable).  Each slice of
+        groups runs its own chain of launches on its own stream; the slices never exchange data."""
+        L = _lib.lib()
         n, M, T, G = b["n"], b["M"], b["T"], self.G
         a = C_ref(self.engine.args)
         par = b["par"]
+        ld = par.shape[1]
         s = b["s"]
+        main = torch.cuda.current_stream()
         s[0].copy_(b["init"])
         ring = self.obs_layout == "ring"
         for k in ("actions", "done", "bad", "gait", "acc", "gbad") + (("ring",) if ring else ("hist",)):
             b[k].zero_()
         S = b["ring"].shape[0] if ring else 0
+        parts = b["parts"]
+        streams = [main] + self._streams[:len(parts) - 1]
+        for x in streams[1:]:
+            x.wait_stream(main)
+        i32 = torch.int32
 
-        def physics(cur: int, sync_first: bool) -> None:
-            rc = L.spi_policy_step(a, _lib.ptr(par), par.shape[1], n, G, _lib.ptr(s[cur]), _lib.ptr(s[1 - cur]),
-                                   _lib.ptr(b["actions"]), _lib.ptr(b["done"], torch.int32), int(sync_first),
-                                   _lib.ptr(b["applied"]), _lib.ptr(b["bad"], torch.int32), st)
+        def physics(p: int, cur: int, sync_first: bool) -> None:
+            g0, g1 = parts[p]
+            e0, e1 = g0 * G, g1 * G
+            rc = L.spi_policy_step(a, par.data_ptr() + 4 * e0, ld, e1 - e0, G, _lib.ptr(s[cur][e0:e1]),
+                                   _lib.ptr(s[1 - cur][e0:e1]), _lib.ptr(b["actions"][e0:e1]),
+                                   _lib.ptr(b["done"][e0:e1], i32), int(sync_first), _lib.ptr(b["applied"][e0:e1]),
+                                   _lib.ptr(b["bad"][e0:e1], i32), _lib.stream())
             _lib.check(rc, "spi_policy_step")
 
-        def observe(cur: int, now: int) -> None:
+        def reward(p: int, cur: int) -> None:
+            g0, g1 = parts[p]
+            e0, e1 = g0 * G, g1 * G
+            rc = L.spi_policy_reward(_lib.ptr(s[cur][e0:e1]), g1 - g0, G, self.delta, self.grav[0], self.grav[1],
+                                     self.termination_rew, _lib.ptr(b["done"][e0:e1], i32),
+                                     _lib.ptr(b["bad"][e0:e1], i32), _lib.ptr(b["acc"][g0:g1]),
+                                     _lib.ptr(b["gbad"][g0:g1], i32), _lib.stream())
+            _lib.check(rc, "spi_policy_reward")
+
+        def observe(p: int, cur: int, now: int) -> None:
+            g0, g1 = parts[p]
+            e0, e1 = g0 * G, g1 * G
             if ring:                     # step `now` writes ring slot now % S
-                rc = L.spi_policy_terms(n, G, _lib.ptr(s[cur]), _lib.ptr(b["cmds"]), T, now, now + 1, self.dt_ctrl,
-                                        _lib.ptr(self.q_default), _lib.ptr(b["gait"]), _lib.ptr(b["applied"]),
-                                        _lib.ptr(b["ring"]), n, now % S, self.evaluating, self.height[0],
-                                        self.height[1], st)
+                rg = b["ring"]
+                rc = L.spi_policy_terms(e1 - e0, G, _lib.ptr(s[cur][e0:e1]), _lib.ptr(b["cmds"][g0:g1]), T, now,
+                                        now + 1, self.dt_ctrl, _lib.ptr(self.q_default), _lib.ptr(b["gait"][e0:e1]),
+                                        _lib.ptr(b["applied"][e0:e1]), rg.data_ptr() + 4 * e0, rg.shape[2], now % S,
+                                        self.evaluating, self.height[0], self.height[1], _lib.stream())
                 _lib.check(rc, "spi_policy_terms")
                 return
-            rc = L.spi_policy_obs(n, G, _lib.ptr(s[cur]), _lib.ptr(b["cmds"]), T, now, now + 1, self.dt_ctrl,
-                                  _lib.ptr(self.q_default), _lib.ptr(b["gait"]), _lib.ptr(b["applied"]),
-                                  _lib.ptr(b["hist"]), _lib.ptr(b["obs"]), self.evaluating, self.height[0],
-                                  self.height[1], self.obs_clip, st)
+            rc = L.spi_policy_obs(e1 - e0, G, _lib.ptr(s[cur][e0:e1]), _lib.ptr(b["cmds"][g0:g1]), T, now, now + 1,
+                                  self.dt_ctrl, _lib.ptr(self.q_default), _lib.ptr(b["gait"][e0:e1]),
+                                  _lib.ptr(b["applied"][e0:e1]), _lib.ptr(b["hist"][e0:e1]),
+                                  _lib.ptr(b["obs"][e0:e1]), self.evaluating, self.height[0], self.height[1],
+                                  self.obs_clip, _lib.stream())
             _lib.check(rc, "spi_policy_obs")
 
-        def act(latest: int) -> None:
+        def act(p: int, latest: int) -> None:
+            g0, g1 = parts[p]
+            e0, e1 = g0 * G, g1 * G
             if ring:
-                self.policy.forward_ring(b["ring"], latest % S, b["kmap"], self.obs_clip, n, out=b["actions"])
+                self.policy.forward_ring(b["ring"], latest % S, b["kmap"], self.obs_clip, e1 - e0,
+                                         out=b["actions"][e0:e1], row0=e0, work=b["work"][p])
             else:
-                self.policy(b["obs"], out=b["actions"])
+                self.policy(b["obs"][e0:e1], out=b["actions"][e0:e1])
 
-        def sync(cur: int, step_idx: int) -> bool:
+        def sync(p: int, cur: int, step_idx: int) -> bool:
             if not self.sync_after(step_idx):
                 return False
             if self.ksync_mode == KSYNC_MODES["state"]:
-                _lib.check(L.spi_policy_sync_state(_lib.ptr(s[cur]), n, G, st), "spi_policy_sync_state")
+                g0, g1 = parts[p]
+                x = s[cur][g0 * G:g1 * G]
+                _lib.check(L.spi_policy_sync_state(_lib.ptr(x), (g1 - g0) * G, G, _lib.stream()),
+                           "spi_policy_sync_state")
                 return False
             return True
 
-        # reset step: zero actions, no reward (base_task.py:90-100)
-        cur = 0
-        physics(cur, False)
+        pending = [False] * len(parts)
+        for p in range(len(parts)):
+            with torch.cuda.stream(streams[p]):
+                # reset step: zero actions, no reward (base_task.py:90-100)
+                physics(p, 0, False)
+                observe(p, 1, 0)
+                pending[p] = sync(p, 1, 1)
         cur = 1
-        observe(cur, 0)
-        pending = sync(cur, 1)
         for it in range(1, T - 1):
-            act(it - 1)
-            physics(cur, pending)
+            for p in range(len(parts)):
+                with torch.cuda.stream(streams[p]):
+                    act(p, it - 1)
+                    physics(p, cur, pending[p])
+                    reward(p, 1 - cur)
+                    observe(p, 1 - cur, it)
+                    pending[p] = sync(p, 1 - cur, it + 1)
             cur = 1 - cur
-            rc = L.spi_policy_reward(_lib.ptr(s[cur]), M, G, self.delta, self.grav[0], self.grav[1],
-                                     self.termination_rew, _lib.ptr(b["done"], torch.int32),
-                                     _lib.ptr(b["bad"], torch.int32), _lib.ptr(b["acc"]),
-                                     _lib.ptr(b["gbad"], torch.int32), st)
-            _lib.check(rc, "spi_policy_reward")
-            observe(cur, it)
-            pending = sync(cur, it + 1)
+        for x in streams[1:]:
+            main.wait_stream(x)
         b["final"] = s[cur]
 
     def evaluate(self, commands: torch.Tensor, init: torch.Tensor, graph: bool = False) -> torch.Tensor:

@@ -40,7 +40,8 @@ def _commands(rng, M, T):
     return c
 
 
-@pytest.mark.parametrize("M,dims", [(1000, [900, 512, 256, 128, 12]), (37, [70, 33, 12]), (5, [64, 1])])
+@pytest.mark.parametrize("M,dims", [(1000, [900, 512, 256, 128, 12]), (37, [70, 33, 12]), (5, [64, 1]),
+                                    (9001, [900, 512, 256, 128, 12]), (8200, [70, 33, 12])])
 def test_mlp_matches_float64(gpu, M, dims):
     """MFMA FP32 layers (ELU between) vs float64 torch on the CPU, ragged tile edges included."""
     from spi_amd.agents.policy import PolicyMLP
@@ -247,16 +248,17 @@ def test_graph_replay_matches_eager(gpu, model, sim, theta0):
     assert not torch.equal(outs[0], outs[1])
 
 
-def test_ring_observation_equals_dense(gpu, model, sim, theta0):
+@pytest.mark.parametrize("M,T", [(40, 40), (4100, 18)])
+def test_ring_observation_equals_dense(gpu, model, sim, theta0, M, T):
     """The ring layout (terms written once, the actor's first layer gathers the 900-wide row) and
     the dense layout (materialised observation + shifted history) give bit-identical rollouts:
-    the gathered operands are the same floats in the same K order."""
+    the gathered operands are the same floats in the same K order.  40 steps wrap the 15-slot
+    ring twice; 8200 envs take the two-row-block tiles."""
     from spi_amd.agents.policy import PolicyFIMRollout, PolicyMLP, reset_states
     from spi_amd.rollout import RolloutEngine
     s = _policy_sim(sim)
     eng = RolloutEngine(s)
     rng = np.random.default_rng(21)
-    M, T = 40, 40                                   # 40 steps: the 15-slot ring wraps twice
     net = PolicyMLP.random(900, 12, seed=4)
     init = _dev(reset_states(M * 2, s.q0, seed=3))
     cmds = _dev(_commands(rng, M, T))
@@ -266,3 +268,30 @@ def test_ring_observation_equals_dense(gpu, model, sim, theta0):
         out[layout] = (roll.evaluate(cmds, init).clone(), roll.final_states().clone())
     assert torch.equal(torch.nan_to_num(out["ring"][0], nan=-7.0), torch.nan_to_num(out["dense"][0], nan=-7.0))
     assert torch.equal(out["ring"][1], out["dense"][1])
+
+
+def test_stream_slices_equal_one_stream(gpu, model, sim, theta0):
+    """Slices of the groups on separate streams (physics of one slice overlapping the actor
+    GEMMs of another) give the same bits as one stream, eagerly and from a captured graph; the
+    5-group batch makes uneven, unaligned slices."""
+    from spi_amd.agents.policy import PolicyFIMRollout, PolicyMLP, reset_states
+    from spi_amd.rollout import RolloutEngine
+    s = _policy_sim(sim)
+    eng = RolloutEngine(s)
+    rng = np.random.default_rng(23)
+    net = PolicyMLP.random(900, 12, seed=5)
+    for M, T in ((64, 24), (5, 20)):
+        init = _dev(reset_states(M * 3, s.q0, seed=M))
+        cmds = _dev(_commands(rng, M, T))
+        ref = None
+        for n_streams in (1, 2, 3):
+            roll = PolicyFIMRollout(eng, net, theta0, ["mass", "comx"], n_streams=n_streams)
+            for graph in (False, True):
+                fim = roll.evaluate(cmds, init, graph=graph).clone()
+                fin = roll.final_states().clone()
+                torch.cuda.synchronize()
+                if ref is None:
+                    ref = (fim, fin)
+                    continue
+                assert torch.equal(torch.nan_to_num(fim, nan=-7.0), torch.nan_to_num(ref[0], nan=-7.0)), (M, n_streams)
+                assert torch.equal(fin, ref[1]), (M, n_streams, graph)

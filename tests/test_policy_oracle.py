@@ -193,3 +193,18 @@ def test_default_param_rows_follow_the_reference_setters():
     with open(os.path.join(here, "..", "spi-active_amd", "spi_amd", "configs", "algo", "active_sysid.yaml")) as f:
         algo = yaml.safe_load(f)["algo"]["config"]
     assert algo["command"]["command_sampling_idxs"] == [0, 2, 5]
+
+
+def test_stream_slices_cover_the_groups():
+    """Host logic of the stream slices: contiguous cover of [0, M), Python ints (they become
+    pointer offsets), slice starts on 4-env boundaries whenever the group count allows."""
+    import types
+    from spi_amd.agents.policy import PolicyFIMRollout
+    for G in (2, 3, 4, 5):
+        for P in (1, 2, 3):
+            for M in (1, 2, 5, 64, 1024, 8191):
+                parts = PolicyFIMRollout._parts(types.SimpleNamespace(G=G, n_streams=P), M)
+                assert parts[0][0] == 0 and parts[-1][1] == M and len(parts) <= P
+                assert all(a < b and type(a) is int and type(b) is int for a, b in parts)
+                assert all(parts[i][1] == parts[i + 1][0] for i in range(len(parts) - 1))
+                assert all((a * G) % 4 == 0 for a, _ in parts)
