@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--policy-batches", default="1024,8192",
                    help="policy: trials per iteration at the reference batch[,wide batch]")
     p.add_argument("--obs-layout", default="ring", choices=["ring", "dense"], help="policy: observation layout")
+    p.add_argument("--grad-shape", default="256,20",
+                   help="gradient: segments,horizon of the rollouts differentiated (4096 samples each)")
     p.add_argument("--profile-launch", action="store_true",
                    help="run one rollout_cost launch of the workload and a 256 MiB device copy (for rocprofv3 "
                         "--pmc FETCH_SIZE / WRITE_SIZE calibration), then exit")
@@ -74,6 +76,9 @@ def main():
         from spi_amd import workloads
         if args.workload == "serve":
             out = workloads.serve(n_requests=args.requests)
+        elif args.workload == "gradient":
+            n_seg, horizon = (int(x) for x in args.grad_shape.split(","))
+            out = workloads.gradient(args.steps, args.warmup, n_seg=n_seg, horizon=horizon)
         elif args.workload == "policy":
             bs = [int(x) for x in args.policy_batches.split(",")]
             out = workloads.policy(args.steps, args.warmup, batch=bs[0], wide_batch=bs[1] if len(bs) > 1 else 0,

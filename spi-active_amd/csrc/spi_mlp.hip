@@ -4,8 +4,8 @@
 // bias and ELU in the epilogue.  linear_kernel: a workgroup of 4 wavefronts owns a 32-row x
 // 128-column output tile (many workgroups for small batches); linear_big_kernel: 128 x 128
 // (a 64 x 64 quadrant per wavefront) for batches of 8192 rows and more.  Rows of X and the
-// weight rows are staged through LDS in K chunks of 64, padded to 65 words per row so the
-// column-wise operand reads hit distinct banks.  Strict IEEE float32 (no fast-math).
+// weight rows are staged through LDS in K chunks of 64 (even columns, then odd ones, 65 words per
+// row, so the operand reads of a wavefront hit 64 distinct banks).  Strict IEEE float32 (no fast-math).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -24,6 +24,11 @@ constexpr int kPad = kKc + 1;
 constexpr int kMaxLayers = 8;
 constexpr int kMaxRingK = 1024;  // widest ring-gathered input
 constexpr int kBigM = 8192;      // batches from here on use linear_big_kernel (N >= 256)
+
+// LDS slot of tile element (row, k): the even columns of a row first, then the odd ones, so
+// the MFMA operand reads of a wavefront (lane -> row l % 32, column kk + l / 32) cover 64
+// distinct banks: row * 65 + 32 (l / 32) + kk / 2.
+__device__ __forceinline__ int lds_at(int row, int k) { return row * kPad + ((k & 1) << 5) + (k >> 1); }
 
 // Layer input gathered from a ring of per-step observation terms, stored term-major
 // ring[slot][term][m] (m = row, fastest): X[m][k] = clip(ring[slot_k][term_k][m]) with
@@ -133,30 +138,30 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
 #pragma unroll
     for (int i = 0; i < kAPer; ++i) {
       if (kSrc == 2) {
-        As[(tid % kRows) * kPad + kk_r + kGStep * i] = ra[i][0];
+        As[lds_at(tid % kRows, kk_r + kGStep * i)] = ra[i][0];
       } else if (kSrc == 3) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) As[(m4 + v) * kPad + kk4 + kG4Step * i] = ra[i][v];
+        for (int v = 0; v < 4; ++v) As[lds_at(m4 + v, kk4 + kG4Step * i)] = ra[i][v];
       } else {
 #pragma unroll
-        for (int v = 0; v < kAV; ++v) As[(r0 + i * kRowStep) * kPad + c0 + v] = ra[i][v];
+        for (int v = 0; v < kAV; ++v) As[lds_at(r0 + i * kRowStep, c0 + v)] = ra[i][v];
       }
     }
 #pragma unroll
     for (int i = 0; i < kWPer; ++i)
 #pragma unroll
-      for (int v = 0; v < kV; ++v) Ws[(r0 + i * kRowStep) * kPad + c0 + v] = rw[i][v];
+      for (int v = 0; v < kV; ++v) Ws[lds_at(r0 + i * kRowStep, c0 + v)] = rw[i][v];
     __syncthreads();
     if (k0 + kKc < K) fetch(k0 + kKc);
     if (live) {
-      const float* ap = As + col * kPad + half;
-      const float* bp = Ws + (wave * 32 + col) * kPad + half;
+      const float* ap = As + col * kPad + 32 * half;
+      const float* bp = Ws + (wave * 32 + col) * kPad + 32 * half;
 #pragma unroll
       for (int kk = 0; kk < kKc; kk += 2) {
-        const float bv = bp[kk];
+        const float bv = bp[kk / 2];
 #pragma unroll
         for (int q = 0; q < kRB; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[q * 32 * kPad + kk], bv, acc[q], 0, 0, 0);
+          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[q * 32 * kPad + kk / 2], bv, acc[q], 0, 0, 0);
       }
     }
     __syncthreads();
@@ -266,31 +271,31 @@ __global__ __launch_bounds__(256) void linear_big_kernel(const float* __restrict
   for (int k0 = 0; k0 < K; k0 += kKc) {
     if (kSrc == 2) {
 #pragma unroll
-      for (int i = 0; i < kGPer; ++i) As[(tid & 127) * kPad + kk_r + 2 * i] = ra[i];
+      for (int i = 0; i < kGPer; ++i) As[lds_at(tid & 127, kk_r + 2 * i)] = ra[i];
     } else if (kSrc == 3) {
 #pragma unroll
       for (int i = 0; i < kPer; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) As[(m4 + v) * kPad + kk4 + 8 * i] = ra[4 * i + v];
+        for (int v = 0; v < 4; ++v) As[lds_at(m4 + v, kk4 + 8 * i)] = ra[4 * i + v];
     } else {
 #pragma unroll
       for (int i = 0; i < kPer; ++i)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) As[(r0 + 16 * i) * kPad + c0 + v] = ra[4 * i + v];
+        for (int v = 0; v < 4; ++v) As[lds_at(r0 + 16 * i, c0 + v)] = ra[4 * i + v];
     }
 #pragma unroll
     for (int i = 0; i < kPer; ++i)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[(r0 + 16 * i) * kPad + c0 + v] = rw[4 * i + v];
+      for (int v = 0; v < 4; ++v) Ws[lds_at(r0 + 16 * i, c0 + v)] = rw[4 * i + v];
     __syncthreads();
     if (k0 + kKc < K) fetch(k0 + kKc);
     if (live) {
-      const float* ap = As + (wr + col) * kPad + half;
-      const float* bp = Ws + (wc + col) * kPad + half;
+      const float* ap = As + (wr + col) * kPad + 32 * half;
+      const float* bp = Ws + (wc + col) * kPad + 32 * half;
 #pragma unroll
       for (int kk = 0; kk < kKc; kk += 2) {
-        const float a0 = ap[kk], a1 = ap[32 * kPad + kk];
-        const float b0 = bp[kk], b1 = bp[32 * kPad + kk];
+        const float a0 = ap[kk / 2], a1 = ap[32 * kPad + kk / 2];
+        const float b0 = bp[kk / 2], b1 = bp[32 * kPad + kk / 2];
         acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
         acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
         acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);

@@ -340,7 +340,7 @@ def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, h
             "value": round(rollouts / (adj_ms / 1e3), 1), "unit": "rollouts/s", "n_gpus": 1, "steps": steps,
             "warmup": warmup, "ms_per_step": round(adj_ms, 2), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-            "config": {"workload": "gradient_4096x256_h20", "samples": n_samples, "segments": n_seg,
+            "config": {"workload": f"gradient_{n_samples}x{n_seg}_h{horizon}", "samples": n_samples, "segments": n_seg,
                        "horizon": horizon, "checkpoint_every": checkpoint_every},
             "adjoint_ms": round(adj_ms, 2), "primal_ms": round(primal_ms, 2),
             "adjoint_over_primal": round(adj_ms / primal_ms, 2),
