@@ -180,3 +180,61 @@ def test_problem_table_packing_matches_c_layout():
         assert prefix[i].item() == tot
         tot += p.n_seg * ((s.n_samples + 127) // 128)
     assert bytes(table.numpy()) == bytes(arr) and n_tasks == tot
+
+
+def test_policy_obs_kmap_follows_the_oracle_layout():
+    """spi_policy_obs_kmap (host table of the ring gather) against the oracle's assembly: with
+    step s holding terms s + term / 100, the oracle's observation after step 20 has in column k
+    the term kmap[k][1] of the step kmap[k][0] steps back (0 before the first step)."""
+    import torch  # noqa: F401
+    import spio_policy as op
+    from spi_amd import build
+    L = ctypes.CDLL(build.ensure_built())
+    km = (ctypes.c_int * 1800)()
+    assert L.spi_policy_obs_kmap(km) == 0
+    kmap = np.array(list(km)).reshape(900, 2)
+    assert L.spi_policy_ring_slots() == op.HISTORY + 1
+    keys = sorted(op.TERMS)
+    offs = np.cumsum([0] + [op.TERMS[k][0] for k in keys])
+    hist = op.empty_history()
+    last = 20
+    for step in range(last + 1):
+        t = {k: step + (offs[i] + np.arange(op.TERMS[k][0])) / 100.0 for i, k in enumerate(keys)}
+        obs, hist = op.assemble(t, hist)
+    back, term = kmap[:, 0], kmap[:, 1]
+    assert back.min() == 0 and back.max() == op.HISTORY and term.min() == 0 and term.max() == 59
+    want = np.where(last - back >= 0, (last - back) + term / 100.0, 0.0)
+    assert np.allclose(obs, want, atol=1e-12)
+
+
+def test_policy_entry_points_reject_bad_arguments():
+    """Argument checks run before any device work: null pointers, ragged groups and aliased
+    state buffers come back as SPI_ERR_INVALID_ARG with a message (no GPU needed)."""
+    import torch  # noqa: F401
+    from spi_amd import _lib, build
+    from spi_amd.params import SimParams
+    L = ctypes.CDLL(build.ensure_built())
+    L.spi_last_error.restype = ctypes.c_char_p
+    args = _lib.sim_args(SimParams())
+    vp = ctypes.c_void_p
+    fake = vp(4096)
+    # ragged: 7 envs in groups of 2
+    assert L.spi_policy_step(ctypes.byref(args), fake, 7, 7, 2, fake, vp(8192), fake, fake, 0, fake, fake, None) == 1
+    assert b"spi_policy_step" in L.spi_last_error()
+    # aliased state buffers
+    assert L.spi_policy_step(ctypes.byref(args), fake, 8, 8, 2, fake, fake, fake, fake, 0, fake, fake, None) == 1
+    # reward needs groups of at least 2 (a main and an aux env)
+    assert L.spi_policy_reward(fake, 4, 1, ctypes.c_float(0.1), ctypes.c_float(0.8), ctypes.c_float(0.8),
+                               ctypes.c_float(0.0), fake, fake, fake, fake, None) == 1
+    # commands index past the sequence
+    assert L.spi_policy_terms(8, 2, fake, fake, 10, 9, 10, ctypes.c_float(0.02), fake, fake, fake, fake, 8, 0, 1,
+                              ctypes.c_float(-0.25), ctypes.c_float(0.15), None) == 1
+    # ring head outside the ring
+    assert L.spi_policy_terms(8, 2, fake, fake, 10, 1, 2, ctypes.c_float(0.02), fake, fake, fake, fake, 8, 15, 1,
+                              ctypes.c_float(-0.25), ctypes.c_float(0.15), None) == 1
+    dims = (ctypes.c_int * 3)(70, 33, 12)
+    # work buffer too small / ring too narrow
+    assert L.spi_mlp_forward(fake, 10, 2, dims, fake, fake, ctypes.c_longlong(10), fake, None) == 1
+    assert L.spi_mlp_forward_ring(fake, 8, 15, 60, 0, fake, ctypes.c_float(100.0), 10, 2, dims, fake, fake,
+                                  ctypes.c_longlong(1000), fake, None) == 1
+    assert b"ring" in L.spi_last_error()
