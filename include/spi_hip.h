@@ -359,6 +359,14 @@ int spi_policy_step(const SpiSimArgs* sim, const float* params_dev, int ld, int 
                     const float* states_in_dev, float* states_out_dev, const float* actions_dev, const int* done_dev,
                     int sync_first, float* applied_dev, int* bad_dev, hipStream_t stream);
 
+/* The same step with four lanes per env, one per leg (the base replicated, the legs' articulated
+ * inertias summed across the four lanes): about a third of the instructions per lane on the
+ * critical path, for batches too small to fill the GPU with one env per lane.  Same arguments
+ * and results up to float rounding (the order of the legs' sums differs). */
+int spi_policy_step_legs(const SpiSimArgs* sim, const float* params_dev, int ld, int n_env, int group,
+                         const float* states_in_dev, float* states_out_dev, const float* actions_dev,
+                         const int* done_dev, int sync_first, float* applied_dev, int* bad_dev, hipStream_t stream);
+
 /* Per group: gravity termination OR-ed over the group into done_dev of every member
  * (legged_robot_base.py:336-339, active_sysid_openloop.py:259-272); acc_dev[g] += termination_rew
  * for a terminated group, else ||X||_F^2 of the finite-difference states

@@ -183,17 +183,21 @@ SPI_HD F6 crf(const M6& v, const F6& f) { return F6{cross(v.w, f.n) + cross(v.v,
 // ------------------------------------------------------------------ joint transforms
 // X = [E 0; -E r^ E] with E = R_k(q)^T and r the constant joint origin (parent coords).
 // motion parent -> child
-template <int K> SPI_HD M6 xm(const M6& m, CV3 r, Real c, Real s) {
+// The origin r is a compile-time CV3 (per-leg code) or a runtime V3 whose structural zeros are
+// literal (leg-parallel code, spi_dyn_leg.h); both fold the zero terms.
+SPI_HD V3 crossc(V3 r, V3 b) { return cross(r, b); }
+SPI_HD V3 cvec(V3 c) { return c; }
+template <int K, class O = CV3> SPI_HD M6 xm(const M6& m, O r, Real c, Real s) {
   return M6{rotT<K>(m.w, c, s), rotT<K>(m.v - crossc(r, m.w), c, s)};
 }
 // force child -> parent: X^T f = [E^T n + r x E^T f; E^T f]
-template <int K> SPI_HD F6 xtf(const F6& f, CV3 r, Real c, Real s) {
+template <int K, class O = CV3> SPI_HD F6 xtf(const F6& f, O r, Real c, Real s) {
   V3 ff = rot<K>(f.f, c, s);
   return F6{rot<K>(f.n, c, s) + crossc(r, ff), ff};
 }
 // inertia child -> parent: X^T I X = T^T (R6^T I R6) T, with T the pure translation by r:
 //   A' = A - B r^ + r^ B^T - r^ C r^,  B' = B + r^ C,  C' = C    (rotated blocks)
-template <int K> SPI_HD ABI xtix(const ABI& I, CV3 r, Real c, Real s) {
+template <int K, class O = CV3> SPI_HD ABI xtix(const ABI& I, O r, Real c, Real s) {
   const Real c2 = c * c - s * s, s2 = 2.0f * c * s;
   S3 A = rot_sym<K>(I.A, c, s, c2, s2);
   G3 B = rot_gen<K>(I.B, c, s, c2, s2);

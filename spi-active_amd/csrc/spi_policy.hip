@@ -13,6 +13,7 @@
 #include <cstdio>
 
 #include "spi_dyn.h"
+#include "spi_dyn_leg.h"
 #include "spi_launch.h"
 
 using namespace spi;
@@ -109,6 +110,69 @@ __global__ __launch_bounds__(kPolThreads) void policy_step_kernel(SpiSimArgs a, 
   if (!finite_bits(nrm) || nrm >= a.state_limit) bad[i] = 1;
   float* dst = s_out + static_cast<size_t>(i) * SPI_STATE_DIM;
   state_to_root(st, dst, dst + 13, dst + 25, 1);
+}
+
+// policy_step_kernel with four lanes per env (lane % 4 = leg, spi_dyn_leg.h): the same step,
+// for batches too small to fill the SIMDs with one env per lane.
+__device__ __forceinline__ float pick4(int l, float x0, float x1, float x2, float x3) {
+  return l == 0 ? x0 : (l == 1 ? x1 : (l == 2 ? x2 : x3));
+}
+__global__ __launch_bounds__(kPolThreads) void policy_step_leg_kernel(SpiSimArgs a, const float* __restrict__ par,
+                                                                      int ld, int n, int group,
+                                                                      const float* __restrict__ s_in,
+                                                                      float* __restrict__ s_out,
+                                                                      const float* __restrict__ actions,
+                                                                      const int* __restrict__ done, int sync_first,
+                                                                      float* __restrict__ applied,
+                                                                      int* __restrict__ bad) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t >> 2, l = t & 3;
+  if (i >= n) return;  // the four lanes of an env leave together
+  LegParams lp;
+  lp.base = make_base_inertia(par[0 * ld + i], par[1 * ld + i], par[2 * ld + i], par[3 * ld + i], par[4 * ld + i],
+                              par[5 * ld + i], par[6 * ld + i], par[7 * ld + i], par[8 * ld + i], par[9 * ld + i]);
+  lp.kp_s = par[(10 + l) * ld + i];
+  lp.kd_s = par[(14 + l) * ld + i];
+  lp.fric = par[(18 + l) * ld + i];
+#pragma unroll
+  for (int g = 0; g < 3; ++g) lp.motor[g] = par[(22 + g) * ld + i];
+  const LegK K = leg_constants(l);
+  float kp[3], kd[3], q0[3], eff[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    kp[k] = pick4(l, a.kp[k], a.kp[3 + k], a.kp[6 + k], a.kp[9 + k]);
+    kd[k] = pick4(l, a.kd[k], a.kd[3 + k], a.kd[6 + k], a.kd[9 + k]);
+    q0[k] = pick4(l, a.q0[k], a.q0[3 + k], a.q0[6 + k], a.q0[9 + k]);
+    eff[k] = pick4(l, a.effort[k], a.effort[3 + k], a.effort[6 + k], a.effort[9 + k]);
+  }
+  LegState st;
+  leg_state_from_root(st, s_in + static_cast<size_t>(i) * SPI_STATE_DIM, l);
+  float act[3];
+  const bool stop = done[i] != 0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float x = stop ? 0.0f : actions[static_cast<size_t>(i) * 12 + 3 * l + k];
+    act[k] = fminf(fmaxf(x, -a.action_clip), a.action_clip);
+    applied[static_cast<size_t>(i) * 12 + 3 * l + k] = act[k];
+  }
+  const int m = i - i % group;
+  const bool view = sync_first && i != m;
+  for (int d = 0; d < a.decimation; ++d) {
+    float tau[3];
+    if (d == 0 && view) {
+      const float* ms = s_in + static_cast<size_t>(m) * SPI_STATE_DIM;
+      float vq[3], vqd[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) { vq[k] = ms[13 + 3 * l + k]; vqd[k] = ms[25 + 3 * l + k]; }
+      control_torques_leg(a, lp, kp, kd, q0, eff, act, vq, vqd, tau);
+    } else {
+      control_torques_leg(a, lp, kp, kd, q0, eff, act, st.q, st.qd, tau);
+    }
+    physics_step_leg(a, lp, K, st, tau);
+  }
+  const float nrm = state_norm_leg(st);
+  if (l == 0 && (!finite_bits(nrm) || nrm >= a.state_limit)) bad[i] = 1;
+  leg_state_to_root(st, s_out + static_cast<size_t>(i) * SPI_STATE_DIM, l);
 }
 
 // Per group: gravity termination of every env (|g_b.x| or |g_b.y| above the limits,
@@ -292,6 +356,23 @@ int spi_policy_step(const SpiSimArgs* sim, const float* params_dev, int ld, int 
       *sim, params_dev, ld, n_env, group, states_in_dev, states_out_dev, actions_dev, done_dev, sync_first, applied_dev,
       bad_dev);
   return plaunch("policy_step_kernel");
+}
+
+int spi_policy_step_legs(const SpiSimArgs* sim, const float* params_dev, int ld, int n_env, int group,
+                         const float* states_in_dev, float* states_out_dev, const float* actions_dev,
+                         const int* done_dev, int sync_first, float* applied_dev, int* bad_dev, hipStream_t stream) {
+  if (!sim || sim->decimation < 1 || sim->decimation > 64 || sim->substeps < 1 || sim->substeps > 64 ||
+      !(sim->dt > 0.0f) || sim->motor_model < 0 || sim->motor_model > 3 || !(sim->state_limit > 0.0f))
+    return pfail("spi_policy_step_legs: invalid SpiSimArgs");
+  if (!groups_ok(n_env, group) || n_env > 0x1fffffff || ld < n_env || !params_dev || !states_in_dev ||
+      !states_out_dev || !actions_dev || !done_dev || !applied_dev || !bad_dev || states_in_dev == states_out_dev)
+    return pfail("spi_policy_step_legs: bad sizes, null pointers or aliased state buffers");
+  if (n_env == 0) return SPI_OK;
+  const long long lanes = 4LL * n_env;
+  policy_step_leg_kernel<<<static_cast<int>((lanes + kPolThreads - 1) / kPolThreads), kPolThreads, 0, stream>>>(
+      *sim, params_dev, ld, n_env, group, states_in_dev, states_out_dev, actions_dev, done_dev, sync_first,
+      applied_dev, bad_dev);
+  return plaunch("policy_step_leg_kernel");
 }
 
 int spi_policy_reward(const float* states_dev, int n_group, int group, float delta, float grav_x, float grav_y,

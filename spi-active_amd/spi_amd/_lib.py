@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],
         "spi_rollout_cost_adjoint": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
         "spi_policy_step": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],
+        "spi_policy_step_legs": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],
         "spi_policy_reward": [vp, ip, ip, fp, fp, fp, fp, vp, vp, vp, vp, vp],
         "spi_policy_obs": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, vp, ip, fp, fp, fp, vp],
         "spi_policy_sync_state": [vp, ip, ip, vp],

@@ -32,6 +32,9 @@ from ..params import PARAM_NAMES
 from ..rollout import C_ref, RolloutEngine
 from .active import KSYNC_MODES, _alias
 
+# Up to this many envs the four-lanes-per-env physics runs faster (one env per lane leaves
+# SIMDs idle below ~64 K envs and has ~3x the instructions per lane on the critical path).
+LEG_PARALLEL_MAX_ENVS = 16384
 OBS_TERMS = 60
 OBS_HISTORY = 14
 ACTOR_HIDDEN = (512, 256, 128)
@@ -166,10 +169,15 @@ class PolicyFIMRollout:
     def __init__(self, engine: RolloutEngine, policy: PolicyMLP, theta: np.ndarray, explore: Sequence[str],
                  delta: float = 0.1, ksync_steps: int = 5, ksync_mode: str = "reference",
                  termination_rew: float = 0.0, gravity_limits=(0.8, 0.8), height_limits=(-0.25, 0.15),
-                 obs_clip: float = 100.0, evaluating: bool = True, obs_layout: str = "ring", n_streams: int = 1):
+                 obs_clip: float = 100.0, evaluating: bool = True, obs_layout: str = "ring", n_streams: int = 1,
+                 physics: str = "auto"):
         if obs_layout not in ("ring", "dense"):
             raise ValueError("obs_layout must be 'ring' or 'dense'")
         self.obs_layout = obs_layout
+        # one env per lane, or four lanes per env (one per leg) when the batch leaves SIMDs idle
+        if physics not in ("auto", "lanes", "legs"):
+            raise ValueError("physics must be 'auto', 'lanes' or 'legs'")
+        self.physics = physics
         # independent slices of the groups on their own streams: one slice's physics (latency-
         # bound, a few wavefronts per SIMD) overlaps another's actor GEMMs (matrix cores)
         self.n_streams = max(1, int(n_streams)) if obs_layout == "ring" else 1
@@ -266,10 +274,13 @@ class PolicyFIMRollout:
             x.wait_stream(main)
         i32 = torch.int32
 
+        legs = self.physics == "legs" or (self.physics == "auto" and n <= LEG_PARALLEL_MAX_ENVS)
+        step_fn = L.spi_policy_step_legs if legs else L.spi_policy_step
+
         def physics(p: int, cur: int, sync_first: bool) -> None:
             g0, g1 = parts[p]
             e0, e1 = g0 * G, g1 * G
-            rc = L.spi_policy_step(a, par.data_ptr() + 4 * e0, ld, e1 - e0, G, _lib.ptr(s[cur][e0:e1]),
+            rc = step_fn(a, par.data_ptr() + 4 * e0, ld, e1 - e0, G, _lib.ptr(s[cur][e0:e1]),
                                    _lib.ptr(s[1 - cur][e0:e1]), _lib.ptr(b["actions"][e0:e1]),
                                    _lib.ptr(b["done"][e0:e1], i32), int(sync_first), _lib.ptr(b["applied"][e0:e1]),
                                    _lib.ptr(b["bad"][e0:e1], i32), _lib.stream())

@@ -92,9 +92,10 @@ def test_obs_kernel_matches_oracle(gpu, sim):
         assert np.abs(gait.cpu().numpy() - g_ref).max() < 1e-6
 
 
-def test_policy_step_matches_oracle(gpu, model, sim, theta0):
+@pytest.mark.parametrize("entry", ["spi_policy_step", "spi_policy_step_legs"])
+def test_policy_step_matches_oracle(gpu, model, sim, theta0, entry):
     """One control step of 2 groups: group 1 terminated (zero actions), aux envs with the
-    view-only joint sync on the first torque."""
+    view-only joint sync on the first torque; one env per lane and four lanes per env."""
     import spio
     from spi_amd import _lib
     from spi_amd.rollout import RolloutEngine
@@ -112,10 +113,10 @@ def test_policy_step_matches_oracle(gpu, model, sim, theta0):
     out, applied, bad = torch.zeros((n, 37), device="cuda"), torch.zeros((n, 12), device="cuda"), _dev(
         np.zeros(n), torch.int32)
     P, states_d, actions_d, done_d = _dev(par.T), _dev(states), _dev(actions), _dev(done, torch.int32)
-    rc = L.spi_policy_step(C.byref(eng.args), _lib.ptr(P), n, n, G, _lib.ptr(states_d), _lib.ptr(out),
+    rc = getattr(L, entry)(C.byref(eng.args), _lib.ptr(P), n, n, G, _lib.ptr(states_d), _lib.ptr(out),
                            _lib.ptr(actions_d), _lib.ptr(done_d, torch.int32), 1,
                            _lib.ptr(applied), _lib.ptr(bad, torch.int32), _lib.stream())
-    _lib.check(rc, "spi_policy_step")
+    _lib.check(rc, entry)
     got, app = out.cpu().numpy(), applied.cpu().numpy()
     assert (bad.cpu().numpy() == 0).all()
     for i in range(n):
@@ -165,7 +166,7 @@ def test_reward_kernel_termination_and_fim(gpu):
         assert a == pytest.approx(0.25 + f, rel=1e-5)
 
 
-def _rollout_case(model, sim, theta0, mode, seed, T, tilt=None):
+def _rollout_case(model, sim, theta0, mode, seed, T, tilt=None, physics="auto"):
     import spio
     import spio_policy as op
     from spi_amd.agents.policy import PolicyFIMRollout, PolicyMLP, reset_states
@@ -180,7 +181,7 @@ def _rollout_case(model, sim, theta0, mode, seed, T, tilt=None):
     with torch.no_grad():
         net.params.mul_(0.5)                      # a calm random policy
     roll = PolicyFIMRollout(eng, net, theta0, ["mass"], delta=0.1, ksync_steps=5, ksync_mode=mode,
-                            termination_rew=-2.0)
+                            termination_rew=-2.0, physics=physics)
     settled, _ = o.simulate(theta0, standing_state(0.32), np.zeros((30, 12), np.float32))
     init = np.repeat(settled[-1][None], M * G, axis=0)
     init[:, 7:13] += reset_states(M * G, s.q0, seed=seed, vel_range=0.1)[:, 7:13]
@@ -198,10 +199,12 @@ def _rollout_case(model, sim, theta0, mode, seed, T, tilt=None):
     return roll, fim, fr, final, done
 
 
-@pytest.mark.parametrize("mode", ["reference", "state"])
-def test_policy_rollout_matches_oracle(gpu, model, sim, theta0, mode):
-    """16 control steps (reset step + 14 policy steps) of 3 groups under a random policy."""
-    roll, fim, fr, final, done = _rollout_case(model, sim, theta0, mode, seed=11, T=16)
+@pytest.mark.parametrize("mode,physics", [("reference", "lanes"), ("state", "lanes"), ("reference", "legs"),
+                                          ("state", "legs")])
+def test_policy_rollout_matches_oracle(gpu, model, sim, theta0, mode, physics):
+    """16 control steps (reset step + 14 policy steps) of 3 groups under a random policy, with
+    one env per lane and with four lanes per env."""
+    roll, fim, fr, final, done = _rollout_case(model, sim, theta0, mode, seed=11, T=16, physics=physics)
     got = roll.final_states().cpu().numpy()
     print(mode, "fim", fim, "oracle", fr, "max state err", np.abs(got - final).max())
     assert np.all(np.isfinite(fim))
@@ -295,3 +298,38 @@ def test_stream_slices_equal_one_stream(gpu, model, sim, theta0):
                     continue
                 assert torch.equal(torch.nan_to_num(fim, nan=-7.0), torch.nan_to_num(ref[0], nan=-7.0)), (M, n_streams)
                 assert torch.equal(fin, ref[1]), (M, n_streams, graph)
+
+
+def test_leg_parallel_step_matches_one_env_per_lane(gpu, sim, theta0):
+    """The four-lanes-per-env kernel against the one-env-per-lane kernel on 1000 random envs in
+    groups of 2 (some terminated, view sync on): same states up to the rounding of the legs'
+    sums, the same applied actions bit for bit and the same divergence flags."""
+    from spi_amd import _lib
+    from spi_amd.rollout import RolloutEngine
+    s = _policy_sim(sim)
+    eng = RolloutEngine(s)
+    L = _lib.lib()
+    rng = np.random.default_rng(17)
+    n, G = 1000, 2
+    par = np.repeat(np.asarray(theta0)[None], n, axis=0)
+    par[:, 0] *= rng.uniform(0.8, 1.2, n)
+    par[:, 10:18] *= rng.uniform(0.8, 1.2, (n, 8))
+    states = np.stack([random_state(rng, s, spread=0.2, vel=0.3) for _ in range(n)]).astype(np.float32)
+    actions = rng.uniform(-3, 3, (n, 12)).astype(np.float32)
+    done = (rng.uniform(size=n) < 0.1).astype(np.int32)
+    P, states_d, actions_d, done_d = _dev(par.T), _dev(states), _dev(actions), _dev(done, torch.int32)
+    res = {}
+    for entry in ("spi_policy_step", "spi_policy_step_legs"):
+        out, applied = torch.zeros((n, 37), device="cuda"), torch.zeros((n, 12), device="cuda")
+        bad = _dev(np.zeros(n), torch.int32)
+        rc = getattr(L, entry)(C.byref(eng.args), _lib.ptr(P), n, n, G, _lib.ptr(states_d), _lib.ptr(out),
+                               _lib.ptr(actions_d), _lib.ptr(done_d, torch.int32), 1, _lib.ptr(applied),
+                               _lib.ptr(bad, torch.int32), _lib.stream())
+        _lib.check(rc, entry)
+        res[entry] = (out.cpu().numpy(), applied.cpu().numpy(), bad.cpu().numpy())
+    a, b = res["spi_policy_step"], res["spi_policy_step_legs"]
+    assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    scale = np.abs(a[0]).max(0) + 1e-3
+    err = (np.abs(a[0] - b[0]) / scale).max()
+    print("max state difference / component scale:", err)
+    assert err < 1e-4
