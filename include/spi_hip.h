@@ -265,6 +265,13 @@ int spi_batch_cem_update(const SpiBatchProblem* probs_dev, int n_prob, hipStream
 int spi_traj_sensitivity(const SpiSimArgs* sim, const SpiTangentSpec* tan, const float* params_dev, int ld,
                          int n_samples, const float* init_dev, const float* actions_dev, int n_exc, int horizon,
                          float* J_dev, float* y_dev, int* status_dev, hipStream_t stream);
+/* The same sensitivities with four lanes per (sample, excitation, tangent), one per leg: the
+ * tangent state of a lane is one leg and the base, which fits the registers (far fewer
+ * spills) at the price of replicating the base work four times.  Same layout and results up to
+ * float rounding (the order of the legs' sums into the base differs). */
+int spi_traj_sensitivity_legs(const SpiSimArgs* sim, const SpiTangentSpec* tan, const float* params_dev, int ld,
+                              int n_samples, const float* init_dev, const float* actions_dev, int n_exc, int horizon,
+                              float* J_dev, float* y_dev, int* status_dev, hipStream_t stream);
 
 /* Fisher information of every group, F_dev[g] = J_g^T J_g ([n_tan][n_tan], n_tan <= 32),
  * with J_g = J_dev[g] of n_rows x n_tan, accumulated with FP32 MFMA (32x32x2). */

@@ -179,6 +179,15 @@ template <int N> SPI_AD_HD Dual<N> rsqrt_d(const Dual<N>& x) {
   return chain(r, -0.5f * r * r * r, x);
 }
 
+// cross-lane exchange of a whole dual number (value and tangents)
+template <int N> __device__ __forceinline__ Dual<N> shfl_xor_lane(const Dual<N>& x, int m) {
+  Dual<N> r;
+  r.v = __shfl_xor(x.v, m, 64);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = __shfl_xor(x.d[i], m, 64);
+  return r;
+}
+
 }  // namespace spi_ad
 
 // the four lane math functions spi_dyn.h calls, for Dual<N>, declared in namespace NS

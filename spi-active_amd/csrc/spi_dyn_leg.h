@@ -43,7 +43,7 @@ constexpr LegTable leg_table() {
   }
   return t;
 }
-__constant__ LegTable kLegTab = leg_table();
+static __constant__ LegTable kLegTab = leg_table();
 
 // This lane's leg: joint origins (runtime values, literal structural zeros), link inertias.
 struct LegK {
@@ -79,10 +79,12 @@ struct LegParams {
   Real kp_s, kd_s, fric, motor[3];
 };
 
-// Sum over the four lanes of an env (lanes 4e .. 4e + 3 of one wavefront).
-__device__ __forceinline__ float sum4(float x) {
-  x += __shfl_xor(x, 1, 64);
-  return x + __shfl_xor(x, 2, 64);
+// Sum over the four lanes of an env (lanes 4e .. 4e + 3 of one wavefront); a scalar type
+// with its own lanes (tangent arithmetic) provides shfl_xor_lane for itself.
+__device__ __forceinline__ float shfl_xor_lane(float x, int m) { return __shfl_xor(x, m, 64); }
+template <class T> __device__ __forceinline__ T sum4(T x) {
+  x += shfl_xor_lane(x, 1);
+  return x + shfl_xor_lane(x, 2);
 }
 __device__ __forceinline__ void sum4(V3& v) { v = V3{sum4(v.x), sum4(v.y), sum4(v.z)}; }
 __device__ __forceinline__ void sum4(ABI& I) {
@@ -269,16 +271,18 @@ __device__ __forceinline__ void leg_state_from_root(LegState& st, const float* s
 #pragma unroll
   for (int k = 0; k < 3; ++k) { st.q[k] = src[13 + 3 * l + k]; st.qd[k] = src[25 + 3 * l + k]; }
 }
+// Root row (IsaacGym layout: world-axis velocities) of the base state.
+template <class OutT> __device__ __forceinline__ void leg_root(const LegState& st, OutT* dst) {
+  G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
+  V3 vw = mul(R, st.v), ww = mul(R, st.w);
+  dst[0] = st.p.x; dst[1] = st.p.y; dst[2] = st.p.z;
+  dst[3] = st.qx; dst[4] = st.qy; dst[5] = st.qz; dst[6] = st.qw;
+  dst[7] = vw.x; dst[8] = vw.y; dst[9] = vw.z;
+  dst[10] = ww.x; dst[11] = ww.y; dst[12] = ww.z;
+}
 // Lane 0 of the env writes the root, every lane its three joints.
-__device__ __forceinline__ void leg_state_to_root(const LegState& st, float* dst, int l) {
-  if (l == 0) {
-    G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
-    V3 vw = mul(R, st.v), ww = mul(R, st.w);
-    dst[0] = st.p.x; dst[1] = st.p.y; dst[2] = st.p.z;
-    dst[3] = st.qx; dst[4] = st.qy; dst[5] = st.qz; dst[6] = st.qw;
-    dst[7] = vw.x; dst[8] = vw.y; dst[9] = vw.z;
-    dst[10] = ww.x; dst[11] = ww.y; dst[12] = ww.z;
-  }
+template <class OutT> __device__ __forceinline__ void leg_state_to_root(const LegState& st, OutT* dst, int l) {
+  if (l == 0) leg_root(st, dst);
 #pragma unroll
   for (int k = 0; k < 3; ++k) { dst[13 + 3 * l + k] = st.q[k]; dst[25 + 3 * l + k] = st.qd[k]; }
 }

@@ -63,11 +63,13 @@ class FisherObjective:
     """D-optimal excitation scoring from exact sensitivities."""
 
     def __init__(self, engine: RolloutEngine, explore: Sequence[str] = IDENTIFIED, reg: float = 1e-6,
-                 scale: Optional[Sequence[float]] = None):
+                 scale: Optional[Sequence[float]] = None, legs: bool = True):
         """scale: per-parameter magnitudes (e.g. the prior mean): the logdet is then that of the
         information about relative parameter changes, comparable across units; None = raw.
-        reg: ridge relative to the mean eigenvalue (trace / P)."""
+        reg: ridge relative to the mean eigenvalue (trace / P).  legs: tangent rollouts with four
+        lanes per tangent, one per leg (spi_traj_sensitivity_legs)."""
         self.engine = engine
+        self.legs = bool(legs)
         self.names = list(explore)
         self.spec = _lib.tangent_spec(_rows(self.names))
         self.P = len(self.names)
@@ -89,9 +91,10 @@ class FisherObjective:
         st = torch.empty(G, dtype=torch.int32, device=dev)
         y = torch.empty((G, H, FIM_ROWS), dtype=torch.float32, device=dev) if with_y else None
         actions = actions.contiguous()   # keep the (possibly copied) operand alive through the launch
-        rc = _lib.lib().spi_traj_sensitivity(C_ref(self.engine.args), C_ref(self.spec), _lib.ptr(theta), S, S,
-                                             _lib.ptr(init), _lib.ptr(actions), E, H, _lib.ptr(J),
-                                             _lib.ptr(y), _lib.ptr(st, torch.int32), _lib.stream())
+        L = _lib.lib()
+        fn = L.spi_traj_sensitivity_legs if self.legs else L.spi_traj_sensitivity
+        rc = fn(C_ref(self.engine.args), C_ref(self.spec), _lib.ptr(theta), S, S, _lib.ptr(init), _lib.ptr(actions),
+                E, H, _lib.ptr(J), _lib.ptr(y), _lib.ptr(st, torch.int32), _lib.stream())
         _lib.check(rc, "spi_traj_sensitivity")
         return J, st, y
 

@@ -221,7 +221,12 @@ def fisher(steps: int, warmup: int, n_samples: int = 4096, n_exc: int = 128, hor
                                       "frac": round(bytes_j / asm_s / 1e9 / HBM_PEAK_GBS, 4),
                                       "useful_tflops": round(flops_asm / asm_s / 1e12, 2)},
             "excitations_per_s": round(n_exc / t, 2),
-            "best_excitation": int(np.argmax(ld)), "logdet_range": [float(ld.min()), float(ld.max())],
+            "best_excitation": int(np.argmax(ld)),
+            "logdet_range": [float(ld[np.isfinite(ld)].min()), float(ld.max())],
+            # excitations whose mean information matrix is not positive definite in FP64 after the
+            # ridge: their sensitivities blow up (|J| ~ 1e10, a chaotic rollout), the FP32 J^T J is
+            # below its own rounding there, and the excitation is scored -inf (never chosen)
+            "not_positive_definite": int((~np.isfinite(ld)).sum()),
             "valid_fraction": round(float(valid.sum()) / G, 4)}
 
 

@@ -210,3 +210,36 @@ def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0):
         RolloutCost.apply(x, eng, seg_dev, 10, w, rows, method, 4).sum().backward()
         torch.cuda.synchronize()
         print(method, "gradient of 64 x 4 x H10:", round((time.perf_counter() - t0) * 1e3, 2), "ms")
+
+
+def test_leg_parallel_sensitivities_match(gpu, sim, theta0):
+    """Tangent rollouts with four lanes per tangent (one per leg) against one lane per tangent
+    (same J up to the rounding of the legs' sums) and against the FP64 oracle, 3 samples x 2
+    excitations x 10 steps x 22 parameters."""
+    from spi_amd.agents.fisher import FisherObjective, fourier_excitations
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    o, init = _settled(sim, theta0)
+    rows = _samples(prior, 3, 1)
+    H = 10
+    acts = fourier_excitations(2, H, sim.dt * sim.decimation, sim.action_scale, seed=4)
+    theta = torch.tensor(np.ascontiguousarray(rows.T), dtype=torch.float32, device="cuda")
+    init_d = torch.tensor(init, device="cuda")
+    J1, st1, y1 = FisherObjective(eng, legs=False).sensitivities(theta, acts, init_d, with_y=True)
+    J4, st4, y4 = FisherObjective(eng, legs=True).sensitivities(theta, acts, init_d, with_y=True)
+    assert torch.equal(st1, st4) and (st4 == 0).all()
+    scale = J1.abs().amax(dim=(0, 1)) + 1e-9
+    rel = ((J4 - J1).abs().amax(dim=(0, 1)) / scale).max().item()
+    print("legs vs one lane per tangent, max |dJ| / column scale:", rel, "max |dy|:", (y4 - y1).abs().max().item())
+    assert rel < 1e-3 and (y4 - y1).abs().max().item() < 1e-4
+    J = J4.cpu().numpy().reshape(2, 3, H, 25, 22)
+    A = acts.cpu().numpy()
+    worst = 0.0
+    for e in range(2):
+        for c in range(3):
+            yr, Jr, rc = o.traj_sens(rows[c], np.arange(22), init, A[e])
+            sc = np.abs(Jr).reshape(-1, 22).max(0) + 1e-9
+            worst = max(worst, float((np.abs(J[e, c] - Jr).reshape(-1, 22).max(0) / sc).max()))
+    assert worst < 2e-2
