@@ -190,6 +190,14 @@ int spi_fd_fim_trace(const float* root_dev, const float* dof_state_dev, const fl
 int spi_rollout_cost(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld, int n_cand,
                      const float* seg_dev, int n_seg, int horizon, float* cost_seg_dev, hipStream_t stream);
 
+/* The same costs with four lanes per (candidate, segment), one per leg (base replicated, the
+ * legs' articulated inertias summed across the four lanes): lower latency for batches that
+ * leave most SIMDs idle, more instructions in total.  Same layout; equal to spi_rollout_cost up
+ * to float rounding (the order of the legs' sums differs). */
+int spi_rollout_cost_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
+                          int n_cand, const float* seg_dev, int n_seg, int horizon, float* cost_seg_dev,
+                          hipStream_t stream);
+
 /* cost_dev[c] = sum_s seg_weight_dev[s] * cost_seg_dev[s * n_cand + c], fixed summation order. */
 int spi_reduce_segments(const float* cost_seg_dev, const float* seg_weight_dev, int n_seg, int n_cand,
                         float* cost_dev, hipStream_t stream);

@@ -261,6 +261,37 @@ __device__ __forceinline__ Real state_norm_leg(const LegState& st) {
   return s + fabsf(st.w.x) + fabsf(st.w.y) + fabsf(st.w.z) + fabsf(st.v.x) + fabsf(st.v.y) + fabsf(st.v.z);
 }
 
+// step_cost of spi_dyn.h: base terms in every lane, joint terms over the four lanes.
+template <class TgtT>
+__device__ __forceinline__ Real step_cost_leg(const SpiCostArgs& c, const LegState& st, const TgtT* tg, int l) {
+  G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
+  V3 vw = mul(R, st.v), ww = mul(R, st.w);
+  Real e0(0.f), e1(0.f), e2(0.f), e3(0.f), e4(0.f), e5(0.f), d;
+  d = st.p.x - tg[0]; e0 += d * d;
+  d = st.p.y - tg[1]; e0 += d * d;
+  d = st.p.z - tg[2]; e0 += d * d;
+  d = st.qx - tg[3]; e1 += d * d;
+  d = st.qy - tg[4]; e1 += d * d;
+  d = st.qz - tg[5]; e1 += d * d;
+  d = st.qw - tg[6]; e1 += d * d;
+  d = vw.x - tg[7]; e2 += d * d;
+  d = vw.y - tg[8]; e2 += d * d;
+  d = vw.z - tg[9]; e2 += d * d;
+  d = ww.x - tg[10]; e3 += d * d;
+  d = ww.y - tg[11]; e3 += d * d;
+  d = ww.z - tg[12]; e3 += d * d;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    d = st.q[k] - tg[13 + 3 * l + k]; e4 += d * d;
+    d = st.qd[k] - tg[25 + 3 * l + k]; e5 += d * d;
+  }
+  e4 = sum4(e4);
+  e5 = sum4(e5);
+  if (c.mode == SPI_COST_TRACKING)
+    return c.w[0] * e0 + c.w[1] * e1 + c.w[2] * e2 + c.w[3] * e3 + c.w[4] * e4 + c.w[5] * e5;
+  return c.w[0] * sqrtf(e0) + c.w[1] * sqrtf(e1) + c.w[4] * sqrtf(e4);
+}
+
 // Root-layout (IsaacGym) state of env row `src` into this lane's LegState.
 __device__ __forceinline__ void leg_state_from_root(LegState& st, const float* src, int l) {
   st.p = V3{src[0], src[1], src[2]};

@@ -6,6 +6,7 @@
 #include <cstring>
 
 #include "spi_dyn.h"
+#include "spi_dyn_leg.h"
 #include "spi_launch.h"
 
 using namespace spi;
@@ -95,6 +96,64 @@ __global__ __launch_bounds__(kRolloutThreads) void rollout_cost_kernel(
   if (c >= n_cand) return;
   const LaneParams lp = load_params(par, ld, c);
   reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] = rollout_lane<1>(a, cost, lp, lds, H);
+}
+
+// rollout_cost_kernel with four lanes per candidate, one per leg (spi_dyn_leg.h): one
+// workgroup = one segment x 64 candidates.  For batches that leave most SIMDs idle (a mass
+// landscape, a small job), the rollout's latency drops at the price of more instructions in
+// total; same costs up to the order of the legs' sums.
+template <class T> __device__ __forceinline__ T pick4(int l, const T& x0, const T& x1, const T& x2, const T& x3) {
+  return l == 0 ? x0 : (l == 1 ? x1 : (l == 2 ? x2 : x3));
+}
+__global__ __launch_bounds__(kRolloutThreads) void rollout_cost_leg_kernel(
+    SpiSimArgs a, SpiCostArgs cost, const float* __restrict__ par, int ld, int n_cand,
+    const float* __restrict__ seg, int H, int stride, int n_cblk, float* __restrict__ out) {
+  extern __shared__ float lds[];
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const int s = logical / n_cblk, cb = logical % n_cblk;
+  const float* src = seg + static_cast<size_t>(s) * stride;
+  for (int i = threadIdx.x; i < stride; i += blockDim.x) lds[i] = src[i];
+  __syncthreads();
+  const int c = cb * (kRolloutThreads / 4) + (threadIdx.x >> 2), l = threadIdx.x & 3;
+  if (c >= n_cand) return;  // the four lanes of a candidate leave together
+  const LaneParams full = load_params(par, ld, c);
+  LegParams lp;
+  lp.base = full.base;
+  lp.kp_s = pick4(l, full.kp_s[0], full.kp_s[1], full.kp_s[2], full.kp_s[3]);
+  lp.kd_s = pick4(l, full.kd_s[0], full.kd_s[1], full.kd_s[2], full.kd_s[3]);
+  lp.fric = pick4(l, full.fric[0], full.fric[1], full.fric[2], full.fric[3]);
+#pragma unroll
+  for (int g = 0; g < 3; ++g) lp.motor[g] = full.motor[g];
+  const LegK K = leg_constants(l);
+  float kp[3], kd[3], q0[3], eff[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    kp[k] = pick4(l, a.kp[k], a.kp[3 + k], a.kp[6 + k], a.kp[9 + k]);
+    kd[k] = pick4(l, a.kd[k], a.kd[3 + k], a.kd[6 + k], a.kd[9 + k]);
+    q0[k] = pick4(l, a.q0[k], a.q0[3 + k], a.q0[6 + k], a.q0[9 + k]);
+    eff[k] = pick4(l, a.effort[k], a.effort[3 + k], a.effort[6 + k], a.effort[9 + k]);
+  }
+  LegState st;
+  leg_state_from_root(st, lds, l);
+  float acc = 0.0f;
+  bool ok = true;
+  for (int t = 0; t < H; ++t) {
+    const float* rec = lds + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+    float act[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) act[k] = rec[3 * l + k];
+    for (int d = 0; d < a.decimation; ++d) {
+      float tau[3];
+      control_torques_leg(a, lp, kp, kd, q0, eff, act, st.q, st.qd, tau);
+      physics_step_leg(a, lp, K, st, tau);
+    }
+    if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost_leg(cost, st, rec + 12, l);
+    const float nrm = state_norm_leg(st);   // identical in the four lanes
+    if (!finite_bits(nrm) || nrm >= a.state_limit) { ok = false; break; }
+  }
+  if (l == 0)
+    reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] =
+        (ok && finite_bits(acc)) ? __float_as_uint(acc) : 0x7f800000u;
 }
 
 // Batched problems: a persistent grid pulls tasks (problem, segment, candidate block) from a
@@ -524,6 +583,27 @@ int spi_rollout_cost(const SpiSimArgs* sim, const SpiCostArgs* cost, const float
   rollout_cost_kernel<<<static_cast<int>(nblk), kRolloutThreads, lds_bytes, stream>>>(
       *sim, *cost, params_dev, ld, n_cand, seg_dev, horizon, stride, n_cblk, cost_seg_dev);
   return launch_check("rollout_cost_kernel");
+}
+
+int spi_rollout_cost_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
+                          int n_cand, const float* seg_dev, int n_seg, int horizon, float* cost_seg_dev,
+                          hipStream_t stream) {
+  if (!sim_args_ok(sim) || !cost || (cost->mode != SPI_COST_TRACKING && cost->mode != SPI_COST_REFERENCE))
+    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost_legs: invalid SpiSimArgs/SpiCostArgs");
+  if (n_cand < 0 || n_seg < 0 || horizon < 1 || ld < n_cand || !params_dev || !seg_dev || !cost_seg_dev)
+    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost_legs: bad sizes or null pointer");
+  if (n_cand == 0 || n_seg == 0) return SPI_OK;
+  const int stride = spi_segment_stride(horizon);
+  const size_t lds_bytes = static_cast<size_t>(stride) * sizeof(float);
+  if (lds_bytes > 64 * 1024) return fail(SPI_ERR_TOO_LARGE, "spi_rollout_cost_legs: horizon too long for LDS staging");
+  const int per_block = kRolloutThreads / 4;
+  const int n_cblk = (n_cand + per_block - 1) / per_block;
+  const long long nblk = static_cast<long long>(n_cblk) * n_seg;
+  if (nblk > 0x7fffffffLL || static_cast<long long>(n_seg) * n_cand > (1LL << 31))
+    return fail(SPI_ERR_TOO_LARGE, "spi_rollout_cost_legs: grid too large");
+  rollout_cost_leg_kernel<<<static_cast<int>(nblk), kRolloutThreads, lds_bytes, stream>>>(
+      *sim, *cost, params_dev, ld, n_cand, seg_dev, horizon, stride, n_cblk, cost_seg_dev);
+  return launch_check("rollout_cost_leg_kernel");
 }
 
 int spi_fim_rollout(const SpiSimArgs* sim, const float* params_dev, int ld, int n_group, int n_param,

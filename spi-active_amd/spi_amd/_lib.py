@@ -105,6 +105,7 @@ def lib() -> C.CDLL:
         "spi_compute_torques": [sa, vp, ip, ip, vp, vp, vp, vp],
         "spi_fd_fim_trace": [vp, vp, vp, ip, ip, fp, vp, vp],
         "spi_rollout_cost": [sa, ca, vp, ip, ip, vp, ip, ip, vp, vp],
+        "spi_rollout_cost_legs": [sa, ca, vp, ip, ip, vp, ip, ip, vp, vp],
         "spi_reduce_segments": [vp, vp, ip, ip, vp, vp],
         "spi_simulate": [sa, vp, ip, ip, vp, vp, ip, vp, vp],
         "spi_fim_rollout": [sa, vp, ip, ip, ip, vp, vp, ip, fp, ip, ip, vp, vp],

@@ -19,6 +19,12 @@ DEFAULT_TRACKING_WEIGHTS = (10.0, 5.0, 1.0, 1.0, 1.0, 0.05)
 REFERENCE_WEIGHTS = (10.0, 5.0, 0.0, 0.0, 1.0, 0.0)
 
 
+# Up to this many (candidate, segment) rollouts the four-lanes-per-rollout kernel is faster:
+# one rollout per lane leaves SIMDs idle below ~64 K rollouts and has ~3x the instructions per
+# lane on the critical path.
+LEG_PARALLEL_MAX_ROLLOUTS = 16384
+
+
 class RolloutEngine:
     """Fused batched rollouts of the Go2 under per-candidate parameters."""
 
@@ -40,8 +46,10 @@ class RolloutEngine:
         return torch.from_numpy(segs.packed()).to(self.device)
 
     def segment_costs(self, params: torch.Tensor, n_cand: int, segs: torch.Tensor, horizon: int,
-                      out: Optional[torch.Tensor] = None) -> torch.Tensor:
-        """Cost of every (segment, candidate) pair: [n_seg, n_cand]."""
+                      out: Optional[torch.Tensor] = None, physics: str = "auto") -> torch.Tensor:
+        """Cost of every (segment, candidate) pair: [n_seg, n_cand].  physics: "lanes" (one
+        rollout per lane), "legs" (four lanes per rollout: lower latency when the batch leaves
+        SIMDs idle) or "auto" (legs up to LEG_PARALLEL_MAX_ROLLOUTS rollouts)."""
         n_seg = segs.shape[0]
         if segs.shape[1] != segment_stride(horizon):
             raise ValueError("segment tensor stride does not match the horizon")
@@ -49,8 +57,12 @@ class RolloutEngine:
             raise ValueError("params must be SoA [25, >= n_cand]")
         if out is None:
             out = torch.empty((n_seg, n_cand), dtype=torch.float32, device=self.device)
-        rc = self.L.spi_rollout_cost(C_ref(self.args), C_ref(self.cargs), _lib.ptr(params), params.shape[1], n_cand,
-                                     _lib.ptr(segs), n_seg, horizon, _lib.ptr(out), _lib.stream())
+        if physics not in ("auto", "lanes", "legs"):
+            raise ValueError("physics must be 'auto', 'lanes' or 'legs'")
+        legs = physics == "legs" or (physics == "auto" and n_cand * n_seg <= LEG_PARALLEL_MAX_ROLLOUTS)
+        fn = self.L.spi_rollout_cost_legs if legs else self.L.spi_rollout_cost
+        rc = fn(C_ref(self.args), C_ref(self.cargs), _lib.ptr(params), params.shape[1], n_cand, _lib.ptr(segs), n_seg,
+                horizon, _lib.ptr(out), _lib.stream())
         _lib.check(rc, "spi_rollout_cost")
         return out
 
