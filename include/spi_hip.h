@@ -351,6 +351,15 @@ int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, con
                              int n_cand, const float* segments_dev, int n_seg, int horizon, int checkpoint_every,
                              float* work_dev, long long work_floats, float* cost_dev, float* grad_dev,
                              hipStream_t stream);
+/* spi_rollout_cost_adjoint with four lanes per rollout (one per leg; base quantities computed
+ * in all four, the legs' contributions summed across them): the same outputs up to the order
+ * of the legs' sums, a quarter of the registers per lane.  work_dev: at least
+ * spi_adjoint_legs_work_floats() floats. */
+long long spi_adjoint_legs_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every);
+int spi_rollout_cost_adjoint_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
+                                  int n_cand, const float* segments_dev, int n_seg, int horizon, int checkpoint_every,
+                                  float* work_dev, long long work_floats, float* cost_dev, float* grad_dev,
+                                  hipStream_t stream);
 
 /* Excitations as Fourier series: actions_dev[e][t][j] = scale * sum_k coeff_dev[e][j][k] *
  * sin(2 pi (k + 1) t dt_ctrl / period), k < n_terms. */

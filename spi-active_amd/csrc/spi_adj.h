@@ -97,8 +97,9 @@ template <int K> SPI_HD Real axis_rot_vjp(const G3& Gbar, Real c, Real s) {
   else D = G3{Real(-1.f) * s, Real(-1.f) * c, z, c, Real(-1.f) * s, z, z, z, z};
   return gdot(Gbar, D);
 }
-// quaternion (xyzw) adjoint of R = quat_to_rot(q) for an incoming Rbar
-SPI_HD void quat_rot_vjp(Real x, Real y, Real z, Real w, const G3& B, StateBar& sb) {
+// quaternion (xyzw) adjoint of R = quat_to_rot(q) for an incoming Rbar (SB: any adjoint state
+// with members qx, qy, qz, qw)
+template <class SB> SPI_HD void quat_rot_vjp(Real x, Real y, Real z, Real w, const G3& B, SB& sb) {
   const Real two(2.f), four(4.f);
   sb.qx += two * (y * B.m01 + z * B.m02 + y * B.m10 - w * B.m12 + z * B.m20 + w * B.m21) -
            four * x * (B.m11 + B.m22);
@@ -124,18 +125,18 @@ SPI_HD void rb_bilinear_vjp(const M6& y, const M6& x, Real scale, RBBar& b) {
   b.ixz += scale * (y.w.x * x.w.z + y.w.z * x.w.x);
   b.iyz += scale * (y.w.y * x.w.z + y.w.z * x.w.y);
 }
-// (m, h, Ibar) of the base (make_base_inertia) -> mass, com, inertia-about-CoM rows 0..9
-SPI_HD void base_inertia_vjp(const RBBar& b, Real m, Real cx, Real cy, Real cz, ParBar& pb) {
+// (m, h, Ibar) of the base (make_base_inertia) -> mass, com, inertia-about-CoM rows 0..9 of g
+SPI_HD void base_inertia_vjp(const RBBar& b, Real m, Real cx, Real cy, Real cz, Real* g) {
   // h = m c (+ const): dh/dm = c, dh/dc = m
   // Ibar_xx = ixx + m (cy^2 + cz^2), Ibar_xy = ixy - m cx cy, ...
-  pb.g[0] += b.m + b.hx * cx + b.hy * cy + b.hz * cz + b.ixx * (cy * cy + cz * cz) + b.iyy * (cx * cx + cz * cz) +
-             b.izz * (cx * cx + cy * cy) - b.ixy * cx * cy - b.ixz * cx * cz - b.iyz * cy * cz;
+  g[0] += b.m + b.hx * cx + b.hy * cy + b.hz * cz + b.ixx * (cy * cy + cz * cz) + b.iyy * (cx * cx + cz * cz) +
+          b.izz * (cx * cx + cy * cy) - b.ixy * cx * cy - b.ixz * cx * cz - b.iyz * cy * cz;
   const Real two(2.f);
-  pb.g[1] += b.hx * m + two * m * cx * (b.iyy + b.izz) - m * (b.ixy * cy + b.ixz * cz);
-  pb.g[2] += b.hy * m + two * m * cy * (b.ixx + b.izz) - m * (b.ixy * cx + b.iyz * cz);
-  pb.g[3] += b.hz * m + two * m * cz * (b.ixx + b.iyy) - m * (b.ixz * cx + b.iyz * cy);
-  pb.g[4] += b.ixx; pb.g[5] += b.iyy; pb.g[6] += b.izz;
-  pb.g[7] += b.ixy; pb.g[8] += b.ixz; pb.g[9] += b.iyz;
+  g[1] += b.hx * m + two * m * cx * (b.iyy + b.izz) - m * (b.ixy * cy + b.ixz * cz);
+  g[2] += b.hy * m + two * m * cy * (b.ixx + b.izz) - m * (b.ixy * cx + b.iyz * cz);
+  g[3] += b.hz * m + two * m * cz * (b.ixx + b.iyy) - m * (b.ixz * cx + b.iyz * cy);
+  g[4] += b.ixx; g[5] += b.iyy; g[6] += b.izz;
+  g[7] += b.ixy; g[8] += b.ixz; g[9] += b.iyz;
 }
 
 // ------------------------------------------------------------------ foot contact adjoint
@@ -466,7 +467,7 @@ SPI_HD void substep_vjp(const SpiSimArgs& a, const LaneParams& lp, const Real* r
   RBBar ib{Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f), Real(0.f)};
   rb_bilinear_vjp(Fb_bar, a_rel, Real(1.f), ib);
   rb_bilinear_vjp(crm(v0, Fb_bar), v0, Real(-1.f), ib);
-  base_inertia_vjp(ib, raw[0], raw[1], raw[2], raw[3], pb);
+  base_inertia_vjp(ib, raw[0], raw[1], raw[2], raw[3], pb.g);
   ob.w = ob.w + v0b.n;
   ob.v = ob.v + v0b.f;
   // rotation adjoint -> quaternion

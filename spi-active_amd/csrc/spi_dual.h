@@ -13,6 +13,8 @@
 
 #include <cmath>
 
+#include "spi_quad.h"
+
 #define SPI_AD_HD __host__ __device__ __forceinline__
 
 namespace spi_ad {
@@ -179,12 +181,12 @@ template <int N> SPI_AD_HD Dual<N> rsqrt_d(const Dual<N>& x) {
   return chain(r, -0.5f * r * r * r, x);
 }
 
-// cross-lane exchange of a whole dual number (value and tangents)
-template <int N> __device__ __forceinline__ Dual<N> shfl_xor_lane(const Dual<N>& x, int m) {
+// exchange of a whole dual number (value and tangents) with lane (this lane xor M) of its quad
+template <int M, int N> __device__ __forceinline__ Dual<N> shfl_xor_lane(const Dual<N>& x) {
   Dual<N> r;
-  r.v = __shfl_xor(x.v, m, 64);
+  r.v = spi_quad_xor<M>(x.v);
 #pragma unroll
-  for (int i = 0; i < N; ++i) r.d[i] = __shfl_xor(x.d[i], m, 64);
+  for (int i = 0; i < N; ++i) r.d[i] = spi_quad_xor<M>(x.d[i]);
   return r;
 }
 

@@ -9,6 +9,7 @@
 // the four legs' sums into the base differs (float rounding).
 #pragma once
 #include "spi_dyn.h"
+#include "spi_quad.h"
 
 namespace SPI_DYN_NS {
 
@@ -79,12 +80,13 @@ struct LegParams {
   Real kp_s, kd_s, fric, motor[3];
 };
 
-// Sum over the four lanes of an env (lanes 4e .. 4e + 3 of one wavefront); a scalar type
-// with its own lanes (tangent arithmetic) provides shfl_xor_lane for itself.
-__device__ __forceinline__ float shfl_xor_lane(float x, int m) { return __shfl_xor(x, m, 64); }
+// Sum over the four lanes of an env (lanes 4e .. 4e + 3 of one wavefront, one DPP quad); a
+// scalar type with its own lanes (tangent arithmetic) provides shfl_xor_lane<M> for itself.
+// The butterfly leaves the same bits in the four lanes (float addition commutes).
+template <int M> __device__ __forceinline__ float shfl_xor_lane(float x) { return spi_quad_xor<M>(x); }
 template <class T> __device__ __forceinline__ T sum4(T x) {
-  x += shfl_xor_lane(x, 1);
-  return x + shfl_xor_lane(x, 2);
+  x += shfl_xor_lane<1>(x);
+  return x + shfl_xor_lane<2>(x);
 }
 __device__ __forceinline__ void sum4(V3& v) { v = V3{sum4(v.x), sum4(v.y), sum4(v.z)}; }
 __device__ __forceinline__ void sum4(ABI& I) {

@@ -1,0 +1,11 @@
+// Lane exchange inside a group of four consecutive lanes (a DPP quad permutation): the value
+// of lane (this lane xor M), M = 1 or 2.  A VALU move with no LDS round trip, for the 4-lane
+// sums of the leg-parallel kernels.  All four lanes of the group must be active.
+#pragma once
+#include <hip/hip_runtime.h>
+
+template <int M> __device__ __forceinline__ float spi_quad_xor(float x) {
+  static_assert(M == 1 || M == 2, "swaps inside a quad: xor 1 or xor 2");
+  constexpr int kCtrl = M == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xF, 0xF, false));
+}

@@ -125,6 +125,7 @@ def lib() -> C.CDLL:
         "spi_command_sequences": [vp, vp, ip, ip, vp, vp],
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],
         "spi_rollout_cost_adjoint": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
+        "spi_rollout_cost_adjoint_legs": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
         "spi_policy_step": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],
         "spi_policy_step_legs": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],
         "spi_policy_reward": [vp, ip, ip, fp, fp, fp, fp, vp, vp, vp, vp, vp],
@@ -135,8 +136,9 @@ def lib() -> C.CDLL:
         "spi_policy_terms": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, ip, ip, ip, fp, fp, vp],
         "spi_policy_obs_kmap": [C.POINTER(ip)],
     }
-    L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
-    L.spi_adjoint_work_floats.restype = C.c_longlong
+    for name in ("spi_adjoint_work_floats", "spi_adjoint_legs_work_floats"):
+        getattr(L, name).argtypes = [ip, ip, ip, ip]
+        getattr(L, name).restype = C.c_longlong
     L.spi_policy_obs_dim.restype = ip
     L.spi_policy_ring_slots.restype = ip
     for name in ("spi_mlp_param_floats", "spi_mlp_work_floats"):

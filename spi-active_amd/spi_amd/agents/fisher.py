@@ -139,25 +139,30 @@ class FisherObjective:
 
 
 def adjoint_cost_grad(engine: RolloutEngine, theta: torch.Tensor, segments: torch.Tensor, horizon: int,
-                      checkpoint_every: int = 10):
-    """Cost [n_seg, n] and gradient [n_seg, n, 25] (all parameter rows) by the reverse-mode
-    kernel ``spi_rollout_cost_adjoint`` with checkpoints every ``checkpoint_every`` control steps."""
+                      checkpoint_every: int = 10, legs: bool = True):
+    """Cost [n_seg, n] and gradient [n_seg, n, 25] (all parameter rows) by reverse mode with
+    checkpoints every ``checkpoint_every`` control steps: four lanes per rollout
+    (``spi_rollout_cost_adjoint_legs``, the default: no register spills, 1.6x faster at
+    4096 x 256 x H 20) or, with ``legs=False``, one lane per rollout
+    (``spi_rollout_cost_adjoint``)."""
     n = theta.shape[1]
     n_seg = segments.shape[0]
     if segments.shape[1] != segment_stride(horizon):
         raise ValueError("segment stride does not match the horizon")
     L = _lib.lib()
-    wf = int(L.spi_adjoint_work_floats(n, n_seg, horizon, checkpoint_every))
+    name = "spi_rollout_cost_adjoint_legs" if legs else "spi_rollout_cost_adjoint"
+    sizer = L.spi_adjoint_legs_work_floats if legs else L.spi_adjoint_work_floats
+    wf = int(sizer(n, n_seg, horizon, checkpoint_every))
     if wf < 0:
         raise ValueError("bad sizes")
     dev = theta.device
     work = torch.empty(max(wf, 1), dtype=torch.float32, device=dev)
     cost = torch.empty((n_seg, n), dtype=torch.float32, device=dev)
     grad = torch.empty((n_seg, n, P_COUNT), dtype=torch.float32, device=dev)
-    rc = L.spi_rollout_cost_adjoint(C_ref(engine.args), C_ref(engine.cargs), _lib.ptr(theta), n, n,
-                                    _lib.ptr(segments), n_seg, horizon, checkpoint_every, _lib.ptr(work), wf,
-                                    _lib.ptr(cost), _lib.ptr(grad), _lib.stream())
-    _lib.check(rc, "spi_rollout_cost_adjoint")
+    rc = getattr(L, name)(C_ref(engine.args), C_ref(engine.cargs), _lib.ptr(theta), n, n, _lib.ptr(segments), n_seg,
+                          horizon, checkpoint_every, _lib.ptr(work), wf, _lib.ptr(cost), _lib.ptr(grad),
+                          _lib.stream())
+    _lib.check(rc, name)
     return cost, grad
 
 

@@ -291,8 +291,9 @@ def serve(n_requests: int = 10000, rate_hz: float = 200.0, budget_gb: float = 20
 def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, horizon: int = 20,
              checkpoint_every: int = 10) -> Dict:
     """Gradient of the tracking cost with respect to all parameters for every (sample, segment)
-    rollout of the 22-parameter CEM workload: reverse mode (checkpoints every 10 control steps)
-    timed over the full 4096 x 256 x H 20, forward mode (22 tangent lanes) on a 1/16 slice."""
+    rollout of the 22-parameter CEM workload: reverse mode (checkpoints every 10 control steps;
+    four lanes per rollout, the default, and one lane per rollout) timed over the full
+    4096 x 256 x H 20, forward mode (22 tangent lanes) on a 1/16 slice."""
     from .agents.fisher import IDENTIFIED, _rows, adjoint_cost_grad
     from . import _lib
     from .rollout import C_ref
@@ -307,17 +308,23 @@ def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, h
     rows = np.clip(rows, prior.lo, prior.hi)
     theta = torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).cuda()
     st = torch.cuda.current_stream()
-    ms = []
-    for rep in range(warmup + steps):
-        a, b = _events(st)
-        a.record(st)
-        cost, grad = adjoint_cost_grad(eng, theta, seg_dev, horizon, checkpoint_every)
-        b.record(st)
-        torch.cuda.synchronize()
-        if rep >= warmup:
-            ms.append(a.elapsed_time(b))
-        del grad
-    adj_ms = float(np.mean(ms))
+
+    def timed(legs: bool) -> float:
+        ms = []
+        for rep in range(warmup + steps):
+            a, b = _events(st)
+            a.record(st)
+            cost, grad = adjoint_cost_grad(eng, theta, seg_dev, horizon, checkpoint_every, legs=legs)
+            b.record(st)
+            torch.cuda.synchronize()
+            if rep >= warmup:
+                ms.append(a.elapsed_time(b))
+            del cost, grad
+        return float(np.mean(ms))
+
+    lane_ms = timed(False)
+    legs_ms = timed(True)
+    adj_ms = legs_ms                       # the default form (four lanes per rollout)
     # primal for the ratio
     a, b = _events(st)
     a.record(st)
@@ -347,7 +354,8 @@ def gradient(steps: int, warmup: int, n_samples: int = 4096, n_seg: int = 256, h
             "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
             "config": {"workload": f"gradient_{n_samples}x{n_seg}_h{horizon}", "samples": n_samples, "segments": n_seg,
                        "horizon": horizon, "checkpoint_every": checkpoint_every},
-            "adjoint_ms": round(adj_ms, 2), "primal_ms": round(primal_ms, 2),
+            "adjoint_ms": round(adj_ms, 2), "adjoint_one_lane_ms": round(lane_ms, 2),
+            "adjoint_four_lanes_ms": round(legs_ms, 2), "primal_ms": round(primal_ms, 2),
             "adjoint_over_primal": round(adj_ms / primal_ms, 2),
             "forward_mode_ms_extrapolated": round(tan_ms_full, 1),
             "forward_mode_over_adjoint": round(tan_ms_full / adj_ms, 2)}

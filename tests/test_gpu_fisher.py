@@ -161,9 +161,11 @@ def test_cost_gradient_matches_oracle_and_central_differences(gpu, sim, theta0):
     assert np.median(rel) < 5e-2 and np.corrcoef(fd, an)[0, 1] > 0.999
 
 
-def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0):
-    """Reverse-mode kernel (checkpoints every 4 control steps, H 10) against the forward-mode
-    kernel on 64 candidates x 4 segments and the FP64 oracle gradient on 3 candidates."""
+@pytest.mark.parametrize("legs", [False, True])
+def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0, legs):
+    """Reverse-mode kernel (checkpoints every 4 control steps, H 10; one or four lanes per
+    rollout) against the forward-mode kernel on 64 candidates x 4 segments and the FP64 oracle
+    gradient on 3 candidates."""
     import time
     import spio
     from spi_amd.agents.fisher import IDENTIFIED, RolloutCost, _rows, adjoint_cost_grad
@@ -177,7 +179,7 @@ def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0):
     seg_dev = eng.upload_segments(segs)
     cand = _samples(prior, 64, 14)
     theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
-    cost_a, grad_a = adjoint_cost_grad(eng, theta, seg_dev, 10, checkpoint_every=4)
+    cost_a, grad_a = adjoint_cost_grad(eng, theta, seg_dev, 10, checkpoint_every=4, legs=legs)
     ref_cost = eng.segment_costs(theta, 64, seg_dev, 10)
     # the reverse-mode kernel replays the primal from its own instantiation of the dynamics
     # (another contraction / scheduling of the same float32 arithmetic): the contact-rich
@@ -210,6 +212,49 @@ def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0):
         RolloutCost.apply(x, eng, seg_dev, 10, w, rows, method, 4).sum().backward()
         torch.cuda.synchronize()
         print(method, "gradient of 64 x 4 x H10:", round((time.perf_counter() - t0) * 1e3, 2), "ms")
+
+
+@pytest.mark.parametrize("cost_mode", ["tracking", "reference"])
+def test_leg_parallel_adjoint_matches(gpu, sim, theta0, cost_mode):
+    """Reverse mode with four lanes per rollout (spi_rollout_cost_adjoint_legs) against one lane
+    per rollout and the FP64 oracle gradient: 70 candidates (a partial workgroup) x 4 segments,
+    H 10, checkpoints every 4; then a state limit that stops every rollout at its first step
+    (+inf cost, zero gradient in both kernels)."""
+    import spio
+    from spi_amd.agents.fisher import adjoint_cost_grad
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.model import load_model
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim, cost_mode=cost_mode)
+    prior = identification_prior(theta0)
+    segs = synthetic_segments(eng.simulate_np, ground_truth(prior, seed=12), sim, n_seg=4, horizon=10, seed=13)
+    seg_dev = eng.upload_segments(segs)
+    cand = _samples(prior, 70, 14)
+    theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
+    c1, g1 = adjoint_cost_grad(eng, theta, seg_dev, 10, checkpoint_every=4, legs=False)
+    c4, g4 = adjoint_cost_grad(eng, theta, seg_dev, 10, checkpoint_every=4, legs=True)
+    assert torch.isfinite(c1).all() and torch.isfinite(c4).all()
+    rel_c = ((c4 - c1).abs() / c1.abs().clamp_min(1e-6)).flatten().cpu().numpy()
+    scale = g1.abs().amax(dim=(0, 1)) + 1e-12                     # per parameter row
+    rel_g = ((g4 - g1).abs() / scale).cpu().numpy()
+    print(cost_mode, "legs vs one lane per rollout: cost rel median / max", np.median(rel_c), rel_c.max(),
+          "gradient / row scale median / p99 / max", np.median(rel_g), np.quantile(rel_g, 0.99), rel_g.max())
+    # same float32 arithmetic up to the order of the legs' sums, amplified by the stiff contact
+    assert np.median(rel_c) < 1e-5 and rel_c.max() < 5e-2
+    assert np.median(rel_g) < 1e-4 and np.quantile(rel_g, 0.99) < 1e-2 and rel_g.max() < 5e-2
+    o = spio.Oracle(load_model(), sim)
+    g = g4.cpu().numpy()
+    for c in range(3):
+        _, gr = o.cost_grad(cand[c], np.arange(25), segs.init, segs.actions, segs.targets,
+                            0 if cost_mode == "tracking" else 1, eng.weights)
+        sc = np.abs(gr).max()
+        assert np.abs(g[:, c, :] - gr).max() < 2e-2 * sc, (c, np.abs(g[:, c, :] - gr).max() / sc)
+    stop = RolloutEngine(sim, cost_mode=cost_mode, state_limit=1e-3)
+    c1, g1 = adjoint_cost_grad(stop, theta, seg_dev, 10, checkpoint_every=4, legs=False)
+    c4, g4 = adjoint_cost_grad(stop, theta, seg_dev, 10, checkpoint_every=4, legs=True)
+    assert torch.isinf(c1).all() and torch.isinf(c4).all()
+    assert (g1 == 0).all() and (g4 == 0).all()
 
 
 def test_leg_parallel_sensitivities_match(gpu, sim, theta0):
