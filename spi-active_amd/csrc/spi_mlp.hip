@@ -41,28 +41,47 @@ struct RingSrc {
   float clip;
 };
 
+// LDS slot of element (row, k) of a tile staged in K chunks of kC: the even columns of a row
+// first, then the odd ones, rows padded to kC + 1 words (lds_at is the kC = 64 case).
+template <int kC> __device__ __forceinline__ int lds_at_c(int row, int k) {
+  return row * (kC + 1) + (k & 1) * (kC / 2) + (k >> 1);
+}
+
 // Y[m][n] = act(sum_k X[m][k] W[n][k] + b[n]); W row-major [N][K] (torch Linear layout).
 // MFMA operands: lane l supplies A[l % 32][k + l / 32] = X-tile element and
 // B[k + l / 32][l % 32] = W[n0 + l % 32][k + l / 32]; accumulator r of lane l holds
 // D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].  The next K chunk is fetched from global memory
 // into registers while the MFMAs consume the current one from LDS.
 // kSrc 0: X row-major, scalar loads; 1: X and W rows 16-byte aligned with K % 4 == 0, float4
-// loads; 2: X gathered from a RingSrc (32 kRB consecutive rows of one column per load), W as in 1.
-template <bool kElu, int kSrc, int kRB>
-__global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M, int K,
-                                                     const float* __restrict__ W, const float* __restrict__ b, int N,
-                                                     float* __restrict__ Y, int ldy) {
-  constexpr int kRows = 32 * kRB;
-  __shared__ float As[kRows * kPad];
-  __shared__ float Ws[kCols * kPad];
+// loads; 2: X gathered from a RingSrc (32 consecutive rows of one column per load), W as in 1;
+// 3: as 2 with float4 loads along the rows.
+// Split K (kS groups of 4 wavefronts, 256 kS threads): group g multiplies chunks g, g + kS,
+// g + 2 kS, ... of K (chunks of 64, of 32 for kS = 4) into its own accumulators from its own
+// LDS tiles; at the end groups 1 .. kS-1 hand theirs to group 0 through LDS, which adds them in
+// group order (a fixed order: the result does not depend on timing) and runs the epilogue.
+// For small batches, whose 32 x 128 tiles leave SIMDs idle or alone: kS wavefronts per SIMD
+// hide each other's load latency and each K loop is kS times shorter.  kS = 1 is the plain
+// tile loop.
+template <bool kElu, int kSrc, int kS>
+__global__ __launch_bounds__(256 * kS) void linear_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M, int K,
+                                                          const float* __restrict__ W, const float* __restrict__ b, int N,
+                                                          float* __restrict__ Y, int ldy) {
+  constexpr int kRows = 32;
+  constexpr int kKs = kS == 4 ? 32 : kKc;              // K chunk of one group
+  constexpr int kPs = kKs + 1;
+  constexpr int kGroupFloats = (kRows + kCols) * kPs;
+  static_assert(kS == 1 || (kS - 1) * 16 * 256 <= kS * kGroupFloats, "reduction buffer fits the tiles");
+  __shared__ float smem[kS * kGroupFloats];
   __shared__ int Kr[kSrc >= 2 ? kMaxRingK : 1];       // ring line of each input column
   const int m0 = blockIdx.x * kRows, n0 = blockIdx.y * kCols;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int grp = threadIdx.x >> 8, tid = threadIdx.x & 255, wave = tid >> 6, lane = tid & 63;
+  float* As = smem + grp * kGroupFloats;
+  float* Ws = As + kRows * kPs;
   const int col = lane & 31, half = lane >> 5;
   const bool live = n0 + wave * 32 < N;  // uniform per wavefront
   if (kSrc >= 2) {
     // any table entry is folded into the ring: slots back in [0, n_slots), term in [0, n_terms)
-    for (int k = tid; k < K; k += 256) {
+    for (int k = threadIdx.x; k < K; k += 256 * kS) {
       const int2 km = rs.kmap[k];
       const int back = min(max(km.x, 0), rs.n_slots - 1);
       const int slot = rs.head >= back ? rs.head - back : rs.head - back + rs.n_slots;
@@ -70,21 +89,20 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
     }
     __syncthreads();
   }
-  floatx16 acc[kRB];
+  floatx16 acc;
 #pragma unroll
-  for (int q = 0; q < kRB; ++q)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[q][r] = 0.0f;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
   constexpr int kV = kSrc == 0 ? 1 : 4;                // floats per W (and row-major X) load
-  constexpr int kTPR = kKc / kV;                       // threads per tile row
-  constexpr int kRowStep = 256 / kTPR;                 // rows covered by one pass of the block
+  constexpr int kTPR = kKs / kV;                       // threads per tile row
+  constexpr int kRowStep = 256 / kTPR;                 // rows covered by one pass of the group
   constexpr int kAV = kSrc == 2 ? 1 : kV;
-  // ring gather: thread -> row (tid % kRows), columns kk_r + i * (256 / kRows); float4 ring
-  // gather (kSrc 3): thread -> rows 4 (tid % (kRows / 4)) .. + 3, columns kk4 + i * (1024 / kRows)
+  // ring gather: thread -> row (tid % 32), columns kk_r + i * 8; float4 ring gather (kSrc 3):
+  // thread -> rows 4 (tid % 8) .. + 3, columns kk4 + i * 32
   constexpr int kGStep = 256 / kRows;
   constexpr int kG4 = kRows / 4, kG4Step = 256 / kG4;
-  constexpr int kAPer = kSrc == 2 ? kRows * kKc / 256 : (kSrc == 3 ? kRows * kKc / 1024 : kRows / kRowStep);
+  constexpr int kAPer = kSrc == 2 ? kRows * kKs / 256 : (kSrc == 3 ? kRows * kKs / 1024 : kRows / kRowStep);
   constexpr int kWPer = kCols / kRowStep;
+  static_assert(kAPer >= 1 && kWPer >= 1, "tile staging covers the chunk");
   const int r0 = tid / kTPR, c0 = (tid % kTPR) * kV;
   const int gm_r = m0 + tid % kRows, kk_r = tid / kRows;
   const int m4 = (tid % kG4) * 4, kk4 = tid / kG4;
@@ -133,54 +151,73 @@ __global__ __launch_bounds__(256) void linear_kernel(const float* __restrict__ X
       }
     }
   };
-  fetch(0);
-  for (int k0 = 0; k0 < K; k0 += kKc) {
+  // every group runs the same number of passes (chunks past K load zeros), so the barriers match
+  const int n_pass = (K + kKs * kS - 1) / (kKs * kS);
+  fetch(grp * kKs);
+  for (int it = 0; it < n_pass; ++it) {
 #pragma unroll
     for (int i = 0; i < kAPer; ++i) {
       if (kSrc == 2) {
-        As[lds_at(tid % kRows, kk_r + kGStep * i)] = ra[i][0];
+        As[lds_at_c<kKs>(tid % kRows, kk_r + kGStep * i)] = ra[i][0];
       } else if (kSrc == 3) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) As[lds_at(m4 + v, kk4 + kG4Step * i)] = ra[i][v];
+        for (int v = 0; v < 4; ++v) As[lds_at_c<kKs>(m4 + v, kk4 + kG4Step * i)] = ra[i][v];
       } else {
 #pragma unroll
-        for (int v = 0; v < kAV; ++v) As[lds_at(r0 + i * kRowStep, c0 + v)] = ra[i][v];
+        for (int v = 0; v < kAV; ++v) As[lds_at_c<kKs>(r0 + i * kRowStep, c0 + v)] = ra[i][v];
       }
     }
 #pragma unroll
     for (int i = 0; i < kWPer; ++i)
 #pragma unroll
-      for (int v = 0; v < kV; ++v) Ws[lds_at(r0 + i * kRowStep, c0 + v)] = rw[i][v];
+      for (int v = 0; v < kV; ++v) Ws[lds_at_c<kKs>(r0 + i * kRowStep, c0 + v)] = rw[i][v];
     __syncthreads();
-    if (k0 + kKc < K) fetch(k0 + kKc);
+    if (it + 1 < n_pass) fetch(((it + 1) * kS + grp) * kKs);
     if (live) {
-      const float* ap = As + col * kPad + 32 * half;
-      const float* bp = Ws + (wave * 32 + col) * kPad + 32 * half;
+      const float* ap = As + col * kPs + (kKs / 2) * half;
+      const float* bp = Ws + (wave * 32 + col) * kPs + (kKs / 2) * half;
 #pragma unroll
-      for (int kk = 0; kk < kKc; kk += 2) {
-        const float bv = bp[kk / 2];
-#pragma unroll
-        for (int q = 0; q < kRB; ++q)
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[q * 32 * kPad + kk / 2], bv, acc[q], 0, 0, 0);
-      }
+      for (int kk = 0; kk < kKs; kk += 2)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(ap[kk / 2], bp[kk / 2], acc, 0, 0, 0);
     }
     __syncthreads();
+  }
+  if (kS > 1) {
+    // the loop ended on a barrier: the tiles are free; groups 1 .. kS-1 -> [kS-1][16][256]
+    float* red = smem;
+    if (grp > 0 && live) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) red[((grp - 1) * 16 + r) * 256 + tid] = acc[r];
+    }
+    __syncthreads();
+    if (grp > 0) return;
+    if (live) {
+      for (int g = 1; g < kS; ++g)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] += red[((g - 1) * 16 + r) * 256 + tid];
+    }
   }
   if (!live) return;
   const int gn = n0 + wave * 32 + col;
   if (gn >= N) return;
   const float bias = b[gn];
 #pragma unroll
-  for (int q = 0; q < kRB; ++q)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int gm = m0 + 32 * q + 8 * (r >> 2) + 4 * half + (r & 3);
-      if (gm < M) {
-        float v = acc[q][r] + bias;
-        if (kElu) v = v > 0.0f ? v : expm1f(v);
-        Y[static_cast<size_t>(gm) * ldy + gn] = v;
-      }
+  for (int r = 0; r < 16; ++r) {
+    const int gm = m0 + 8 * (r >> 2) + 4 * half + (r & 3);
+    if (gm < M) {
+      float v = acc[r] + bias;
+      if (kElu) v = v > 0.0f ? v : expm1f(v);
+      Y[static_cast<size_t>(gm) * ldy + gn] = v;
     }
+  }
+}
+
+// Groups of 4 wavefronts per 32 x 128 tile: split K while the tiles alone would leave SIMDs
+// idle or with one wavefront (kS needs K >= 128: a chunk per group).
+int split_for(int M, int K, int N) {
+  const long long tiles = static_cast<long long>((M + 31) / 32) * ((N + kCols - 1) / kCols);
+  if (K < 128 || tiles >= 512) return 1;
+  return tiles <= 128 ? 4 : 2;
 }
 
 // Large batches: a workgroup of 4 wavefronts owns a 128 x 128 output tile, each wavefront a
@@ -353,7 +390,17 @@ int launch_linear(const float* X, int ldx, const RingSrc& rs, int M, int K, cons
 #undef SPI_BIG
   } else {
     const dim3 grid((M + 31) / 32, (N + kCols - 1) / kCols);
-#define SPI_LINEAR(E, S) linear_kernel<E, S, 1><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
+    const int split = split_for(M, K, N);
+#define SPI_LINEAR_S(E, S, P) linear_kernel<E, S, P><<<grid, 256 * P, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
+#define SPI_LINEAR(E, S)            \
+  do {                              \
+    if (split == 4)                 \
+      SPI_LINEAR_S(E, S, 4);        \
+    else if (split == 2)            \
+      SPI_LINEAR_S(E, S, 2);        \
+    else                            \
+      SPI_LINEAR_S(E, S, 1);        \
+  } while (0)
     if (ring4) {
       if (elu) SPI_LINEAR(true, 3); else SPI_LINEAR(false, 3);
     } else if (rs.ring) {
@@ -364,6 +411,7 @@ int launch_linear(const float* X, int ldx, const RingSrc& rs, int M, int K, cons
       if (elu) SPI_LINEAR(true, 0); else SPI_LINEAR(false, 0);
     }
 #undef SPI_LINEAR
+#undef SPI_LINEAR_S
   }
   const hipError_t e = hipGetLastError();
   if (e == hipSuccess) return SPI_OK;

@@ -40,6 +40,8 @@ def test_batch_result_independent_of_batch_mates(gpu, sim, theta0):
 
 
 def test_batch_rollout_matches_single_kernel(gpu, sim, theta0):
+    """The persistent batched kernel against the single-problem kernel in the same form (one
+    rollout per lane; small single problems otherwise pick four lanes per rollout)."""
     from spi_amd.agents.batch import BatchCEM
     from spi_amd.rollout import RolloutEngine
     eng = RolloutEngine(sim)
@@ -48,7 +50,8 @@ def test_batch_rollout_matches_single_kernel(gpu, sim, theta0):
     b.sample()
     b.rollout()
     for spec, buf in zip(b.specs, b.bufs):
-        ref = eng.segment_costs(buf.theta, spec.n_samples, spec.segments, spec.horizon).cpu().numpy()
+        ref = eng.segment_costs(buf.theta, spec.n_samples, spec.segments, spec.horizon,
+                                physics="lanes").cpu().numpy()
         got = buf.cost_seg.cpu().numpy()
         assert np.allclose(got, ref, rtol=1e-5, atol=1e-7), np.abs(got - ref).max()
 

@@ -41,9 +41,12 @@ def _commands(rng, M, T):
 
 
 @pytest.mark.parametrize("M,dims", [(1000, [900, 512, 256, 128, 12]), (37, [70, 33, 12]), (5, [64, 1]),
-                                    (9001, [900, 512, 256, 128, 12]), (8200, [70, 33, 12])])
+                                    (9001, [900, 512, 256, 128, 12]), (8200, [70, 33, 12]),
+                                    (2048, [900, 512, 256, 128, 12]), (600, [130, 257, 12])])
 def test_mlp_matches_float64(gpu, M, dims):
-    """MFMA FP32 layers (ELU between) vs float64 torch on the CPU, ragged tile edges included."""
+    """MFMA FP32 layers (ELU between) vs float64 torch on the CPU, ragged tile edges included;
+    the shapes cover the K split in 1, 2 and 4 groups (2048 rows: layer 1 in 2 groups with a
+    ragged last chunk; 1000 and 600 rows: 4 groups, 600 with scalar loads for odd K)."""
     from spi_amd.agents.policy import PolicyMLP
     net = PolicyMLP.random(dims[0], dims[-1], hidden=dims[1:-1], seed=3)
     x = torch.randn(M, dims[0], generator=torch.Generator().manual_seed(1)) * 2
