@@ -313,6 +313,13 @@ int spi_residual_jacobian(const SpiSimArgs* sim, const SpiCostArgs* cost, const 
                           const float* params_dev, int ld, int n_cand, const float* segments_dev,
                           const float* seg_weight_dev, int n_seg, int horizon, float* Jr_dev, int* status_dev,
                           hipStream_t stream);
+/* spi_residual_jacobian with four lanes per (group, parameter) tangent, one per leg: the same
+ * matrix up to the order of the legs' sums, for refinement steps whose few hundred groups would
+ * leave most SIMDs idle with one lane per tangent. */
+int spi_residual_jacobian_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
+                               const float* params_dev, int ld, int n_cand, const float* segments_dev,
+                               const float* seg_weight_dev, int n_seg, int horizon, float* Jr_dev, int* status_dev,
+                               hipStream_t stream);
 
 /* Levenberg-Marquardt step per candidate from the augmented Gram matrices gram_dev
  * [n_cand * n_seg][n_tan + 1][n_tan + 1] of spi_residual_jacobian's groups (spi_fim_assemble):

@@ -339,6 +339,98 @@ __global__ __launch_bounds__(kSensThreads) void residual_sens_kernel(SpiSimArgs 
   if (k == 0) status[g] = ok ? 0 : 1;
 }
 
+// residual_sens_kernel with four lanes per (group, tangent) unit, one per leg (spi_dyn_leg.h):
+// unit u = g * (n_tan + 1) + k, lanes 4u .. 4u + 3.  Lane 0 of a unit writes the 13 root rows,
+// lane l its leg's joint rows 13 + 3l .. and 25 + 3l ..; the same matrix as the one-lane kernel
+// up to the order of the legs' sums.  For the few hundred groups of a refinement step, whose
+// one-lane form leaves most SIMDs idle.
+__global__ __launch_bounds__(kSensThreads) void residual_sens_leg_kernel(SpiSimArgs a, SpiCostArgs cost,
+                                                                         SpiTangentSpec tan,
+                                                                         const float* __restrict__ par, int ld,
+                                                                         int n_cand, const float* __restrict__ seg,
+                                                                         const float* __restrict__ seg_w, int n_seg,
+                                                                         int H, int stride, float* __restrict__ Jr,
+                                                                         int* __restrict__ status) {
+  const int P = tan.n_tan, L = P + 1;
+  const long long u = static_cast<long long>(blockIdx.x) * (kSensThreads / 4) + (threadIdx.x >> 2);
+  const int l = threadIdx.x & 3;
+  if (u >= static_cast<long long>(n_cand) * n_seg * L) return;  // whole 4-lane units leave together
+  const long long g = u / L;
+  const int k = static_cast<int>(u % L);
+  const int c = static_cast<int>(g / n_seg), s = static_cast<int>(g % n_seg);
+  const float* rec0 = seg + static_cast<size_t>(s) * stride;
+  const LaneParams full = load_params_tan(par, ld, c, k < P ? tan.param[k] : -1);
+  LegParams lp;
+  lp.base = full.base;
+  lp.kp_s = pick4(l, full.kp_s[0], full.kp_s[1], full.kp_s[2], full.kp_s[3]);
+  lp.kd_s = pick4(l, full.kd_s[0], full.kd_s[1], full.kd_s[2], full.kd_s[3]);
+  lp.fric = pick4(l, full.fric[0], full.fric[1], full.fric[2], full.fric[3]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) lp.motor[q] = full.motor[q];
+  const LegK K = leg_constants(l);
+  float kp[3], kd[3], q0[3], eff[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    kp[j] = pick4(l, a.kp[j], a.kp[3 + j], a.kp[6 + j], a.kp[9 + j]);
+    kd[j] = pick4(l, a.kd[j], a.kd[3 + j], a.kd[6 + j], a.kd[9 + j]);
+    q0[j] = pick4(l, a.q0[j], a.q0[3 + j], a.q0[6 + j], a.q0[9 + j]);
+    eff[j] = pick4(l, a.effort[j], a.effort[3 + j], a.effort[6 + j], a.effort[9 + j]);
+  }
+  LegState st;
+  leg_state_from_root(st, rec0, l);
+  float sw[6];
+  const float ws = seg_w[s];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) sw[q] = sqrtf(fmaxf(cost.w[q] * ws, 0.0f));
+  float* Jg = Jr + static_cast<size_t>(g) * H * SPI_STATE_DIM * L + k;
+  bool ok = true;
+  for (int t = 0; t < H; ++t) {
+    const float* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+    float* row = Jg + static_cast<size_t>(t) * SPI_STATE_DIM * L;
+    if (ok) {
+      DualT act[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) act[j] = DualT(rec[3 * l + j]);
+      for (int d = 0; d < a.decimation; ++d) {
+        DualT tau[3];
+        control_torques_leg(a, lp, kp, kd, q0, eff, act, st.q, st.qd, tau);
+        physics_step_leg(a, lp, K, st, tau);
+      }
+      const float n = state_norm_leg(st).v;   // the same in the four lanes
+      ok = finite_bits(n) && n < a.state_limit;
+    }
+    const float* tg = rec + 12;
+    if (ok) {
+      if (l == 0) {
+        DualT r[13];
+        leg_root(st, r);
+#pragma unroll
+        for (int i = 0; i < 13; ++i) {
+          const int grp = i < 3 ? 0 : (i < 7 ? 1 : (i < 10 ? 2 : 3));
+          row[i * L] = sw[grp] * (k < P ? r[i].d[0] : r[i].v - tg[i]);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int iq = 13 + 3 * l + j, iv = 25 + 3 * l + j;
+        row[iq * L] = sw[4] * (k < P ? st.q[j].d[0] : st.q[j].v - tg[iq]);
+        row[iv * L] = sw[5] * (k < P ? st.qd[j].d[0] : st.qd[j].v - tg[iv]);
+      }
+    } else {
+      if (l == 0) {
+#pragma unroll
+        for (int i = 0; i < 13; ++i) row[i * L] = 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        row[(13 + 3 * l + j) * L] = 0.0f;
+        row[(25 + 3 * l + j) * L] = 0.0f;
+      }
+    }
+  }
+  if (k == 0 && l == 0) status[g] = ok ? 0 : 1;
+}
+
 bool tan_ok(const SpiTangentSpec* t) {
   if (!t || t->n_tan < 1 || t->n_tan > SPI_MAX_TANGENTS || t->n_tan > SPI_P_COUNT) return false;
   for (int k = 0; k < t->n_tan; ++k)
@@ -437,6 +529,30 @@ int spi_residual_jacobian(const SpiSimArgs* sim, const SpiCostArgs* cost, const 
       *sim, *cost, *tan, params_dev, ld, n_cand, segments_dev, seg_weight_dev, n_seg, horizon, stride, gpb, Jr_dev,
       status_dev);
   return slaunch("residual_sens_kernel");
+}
+
+int spi_residual_jacobian_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const SpiTangentSpec* tan,
+                               const float* params_dev, int ld, int n_cand, const float* segments_dev,
+                               const float* seg_weight_dev, int n_seg, int horizon, float* Jr_dev, int* status_dev,
+                               hipStream_t stream) {
+  if (!sim_ok(sim)) return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian_legs: invalid SpiSimArgs");
+  if (!tan_ok(tan) || tan->n_tan + 1 > SPI_MAX_TANGENTS)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian_legs: bad tangent spec (n_tan + 1 <= 32)");
+  if (!cost || cost->mode != SPI_COST_TRACKING)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian_legs: needs the tracking (sum of squares) cost");
+  if (n_cand < 0 || ld < n_cand || n_seg < 0 || horizon < 1 || !params_dev || !segments_dev || !seg_weight_dev ||
+      !Jr_dev || !status_dev)
+    return sfail(SPI_ERR_INVALID_ARG, "spi_residual_jacobian_legs: bad sizes or null pointer");
+  const long long G = static_cast<long long>(n_cand) * n_seg;
+  if (G == 0) return SPI_OK;
+  const int stride = SPI_STATE_DIM + horizon * (12 + SPI_STATE_DIM);
+  const long long units = G * (tan->n_tan + 1);
+  const long long blocks = (units + kSensThreads / 4 - 1) / (kSensThreads / 4);
+  if (blocks > 0x7fffffffLL) return sfail(SPI_ERR_TOO_LARGE, "spi_residual_jacobian_legs: grid too large");
+  residual_sens_leg_kernel<<<static_cast<unsigned>(blocks), kSensThreads, 0, stream>>>(
+      *sim, *cost, *tan, params_dev, ld, n_cand, segments_dev, seg_weight_dev, n_seg, horizon, stride, Jr_dev,
+      status_dev);
+  return slaunch("residual_sens_leg_kernel");
 }
 
 }  // extern "C"

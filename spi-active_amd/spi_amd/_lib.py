@@ -121,6 +121,7 @@ def lib() -> C.CDLL:
         "spi_rollout_cost_grad": [sa, ca, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, ip, ip, vp, vp, vp],
         "spi_fourier_excitations": [vp, ip, ip, ip, fp, fp, fp, vp, vp],
         "spi_residual_jacobian": [sa, ca, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp],
+        "spi_residual_jacobian_legs": [sa, ca, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp],
         "spi_gn_step": [vp, vp, ip, ip, ip, vp, vp, vp, vp, vp],
         "spi_command_sequences": [vp, vp, ip, ip, vp, vp],
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],

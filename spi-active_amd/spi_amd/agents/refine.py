@@ -35,12 +35,14 @@ class RefineResult:
 class GaussNewtonRefiner:
     def __init__(self, engine: RolloutEngine, segments: torch.Tensor, horizon: int,
                  seg_weights: Optional[torch.Tensor] = None, names: Sequence[str] = IDENTIFIED,
-                 lo: Optional[np.ndarray] = None, hi: Optional[np.ndarray] = None):
+                 lo: Optional[np.ndarray] = None, hi: Optional[np.ndarray] = None, legs: bool = True):
         if engine.cost_mode != 0:
             raise ValueError("Gauss-Newton refinement needs the tracking (sum of squares) cost")
         if segments.shape[1] != segment_stride(horizon):
             raise ValueError("segment stride does not match the horizon")
         self.engine, self.segments, self.H = engine, segments, horizon
+        # residual Jacobian with four lanes per tangent (one per leg) or one lane per tangent
+        self.jacobian_entry = "spi_residual_jacobian_legs" if legs else "spi_residual_jacobian"
         self.n_seg = segments.shape[0]
         dev = engine.device
         self.seg_w = seg_weights if seg_weights is not None else \
@@ -72,10 +74,11 @@ class GaussNewtonRefiner:
         """One damped Gauss-Newton step: (delta [P], cost at theta, predicted cost after it)."""
         t = self._theta_dev(theta)
         L = _lib.lib()
-        rc = L.spi_residual_jacobian(C_ref(self.engine.args), C_ref(self.engine.cargs), C_ref(self.spec),
-                                     _lib.ptr(t), 1, 1, _lib.ptr(self.segments), _lib.ptr(self.seg_w), self.n_seg,
-                                     self.H, _lib.ptr(self.Jr), _lib.ptr(self.status, torch.int32), _lib.stream())
-        _lib.check(rc, "spi_residual_jacobian")
+        rc = getattr(L, self.jacobian_entry)(C_ref(self.engine.args), C_ref(self.engine.cargs), C_ref(self.spec),
+                                             _lib.ptr(t), 1, 1, _lib.ptr(self.segments), _lib.ptr(self.seg_w),
+                                             self.n_seg, self.H, _lib.ptr(self.Jr), _lib.ptr(self.status, torch.int32),
+                                             _lib.stream())
+        _lib.check(rc, self.jacobian_entry)
         _lib.check(L.spi_fim_assemble(_lib.ptr(self.Jr), self.n_seg, self.H * 37, self.P + 1, _lib.ptr(self.gram),
                                       _lib.stream()), "spi_fim_assemble")
         self.lam.fill_(lam)

@@ -83,3 +83,47 @@ def test_refine_matches_fp64_oracle_minimiser(gpu, sim, theta0):
     rel = np.abs(gpu_res.theta[:22] - th64[:22]) / np.abs(th64[:22])
     print("max rel param difference (|theta| > 1e-2):", rel[big].max())
     assert rel[big].max() < 1e-2
+
+
+def test_leg_parallel_residual_jacobian_matches(gpu, sim, theta0):
+    """[J | r] with four lanes per tangent (spi_residual_jacobian_legs) against one lane per
+    tangent: 3 candidates x 6 segments x H 5, the 22 identified rows; every element written
+    (the buffers start as NaN); then a state limit that stops every group at its first step
+    (status 1 and zero rows in both)."""
+    from spi_amd import _lib
+    from spi_amd.agents.fisher import IDENTIFIED, _rows
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import C_ref, RolloutEngine
+    prior = identification_prior(theta0)
+    rng = np.random.default_rng(2)
+    rows = np.repeat(prior.mean[None], 3, axis=0)
+    rows[:, prior.free] *= 1 + 0.05 * rng.normal(size=(3, int(prior.free.sum())))
+    rows = np.clip(rows, prior.lo, prior.hi)
+    theta = torch.tensor(np.ascontiguousarray(rows.T), dtype=torch.float32, device="cuda")
+    spec = _lib.tangent_spec(_rows(IDENTIFIED))
+    L = spec.n_tan + 1
+    w = torch.full((6,), 1.0 / 6, device="cuda")
+    for limit in (1.0e4, 1.0e-3):
+        eng = RolloutEngine(sim, state_limit=limit)
+        _, segs = _data(eng, prior, 0.01, 6, 5, 11)
+        seg_dev = eng.upload_segments(segs)
+        out = {}
+        for name in ("spi_residual_jacobian", "spi_residual_jacobian_legs"):
+            Jr = torch.full((18, 5 * 37, L), float("nan"), device="cuda")
+            st = torch.full((18,), -1, dtype=torch.int32, device="cuda")
+            rc = getattr(_lib.lib(), name)(C_ref(eng.args), C_ref(eng.cargs), C_ref(spec), _lib.ptr(theta), 3, 3,
+                                           _lib.ptr(seg_dev), _lib.ptr(w), 6, 5, _lib.ptr(Jr),
+                                           _lib.ptr(st, torch.int32), _lib.stream())
+            _lib.check(rc, name)
+            out[name] = (Jr, st)
+        (J1, s1), (J4, s4) = out["spi_residual_jacobian"], out["spi_residual_jacobian_legs"]
+        assert torch.equal(s1, s4)
+        assert torch.isfinite(J4).all() and torch.isfinite(J1).all()
+        if limit > 1.0:
+            assert (s4 == 0).all()
+            scale = J1.abs().amax(dim=(0, 1)) + 1e-9                  # per column of [J | r]
+            rel = ((J4 - J1).abs().amax(dim=(0, 1)) / scale).max().item()
+            print("residual Jacobian, legs vs one lane per tangent, max |d| / column scale:", rel)
+            assert rel < 1e-3
+        else:
+            assert (s4 == 1).all() and (J4 == 0).all() and (J1 == 0).all()
