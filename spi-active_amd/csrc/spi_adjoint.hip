@@ -216,12 +216,9 @@ int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, con
                              float* work_dev, long long work_floats, float* cost_dev, float* grad_dev,
                              hipStream_t stream) {
   auto fail = [](int code, const char* msg) { spi_set_error(msg); return code; };
-  if (!sim || sim->decimation < 1 || sim->decimation > kMaxDecimation || sim->substeps < 1 ||
-      sim->substeps > kMaxSubsteps || !(sim->dt > 0.0f) || !(sim->state_limit > 0.0f))
-    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost_adjoint: invalid SpiSimArgs (decimation, substeps <= 8)");
-  if (!cost || n_cand < 0 || ld < n_cand || n_seg < 0 || horizon < 1 || checkpoint_every < 1 || !params_dev ||
-      !segments_dev || !cost_dev || !grad_dev || !work_dev)
-    return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost_adjoint: bad sizes or null pointer");
+  const int rc = adjoint_args_check("spi_rollout_cost_adjoint", sim, cost, params_dev, ld, n_cand, segments_dev, n_seg,
+                                    horizon, checkpoint_every, work_dev, cost_dev, grad_dev);
+  if (rc != SPI_OK) return rc;
   if (n_cand == 0 || n_seg == 0) return SPI_OK;
   if (work_floats < spi_adjoint_work_floats(n_cand, n_seg, horizon, checkpoint_every))
     return fail(SPI_ERR_INVALID_ARG, "spi_rollout_cost_adjoint: work buffer smaller than spi_adjoint_work_floats()");

@@ -238,3 +238,49 @@ def test_policy_entry_points_reject_bad_arguments():
     assert L.spi_mlp_forward_ring(fake, 8, 15, 60, 0, fake, ctypes.c_float(100.0), 10, 2, dims, fake, fake,
                                   ctypes.c_longlong(1000), fake, None) == 1
     assert b"ring" in L.spi_last_error()
+
+
+def test_derivative_entry_points_reject_bad_arguments():
+    """The reverse-mode and residual-Jacobian entries (both lane layouts) check their arguments
+    before any device work: work buffers below the size function, decimation above 8, the
+    reference cost for a sum-of-squares Jacobian, and tangent counts above the limit come back as
+    SPI_ERR_INVALID_ARG with a message naming the entry (no GPU needed)."""
+    import torch  # noqa: F401
+    from spi_amd import _lib, build
+    from spi_amd.params import SimParams
+    L = ctypes.CDLL(build.ensure_built())
+    L.spi_last_error.restype = ctypes.c_char_p
+    for name in ("spi_adjoint_work_floats", "spi_adjoint_legs_work_floats"):
+        getattr(L, name).restype = ctypes.c_longlong
+    vp, ll = ctypes.c_void_p, ctypes.c_longlong
+    fake = vp(4096)
+    args = _lib.sim_args(SimParams())
+    cost = _lib.SpiCostArgs(0, (ctypes.c_float * 6)(10, 5, 1, 1, 1, 0.05))
+    # four lanes of 19 floats per rollout against one lane of 37
+    one = L.spi_adjoint_work_floats(64, 4, 20, 10)
+    four = L.spi_adjoint_legs_work_floats(64, 4, 20, 10)
+    assert one == 64 * 4 * (2 + 10) * 37 and four == 64 * 4 * 4 * (2 + 10) * 19
+    for name, need in (("spi_rollout_cost_adjoint", one), ("spi_rollout_cost_adjoint_legs", four)):
+        fn = getattr(L, name)
+        assert fn(ctypes.byref(args), ctypes.byref(cost), fake, 64, 64, fake, 4, 20, 10, fake, ll(need - 1), fake,
+                  fake, None) == 1
+        assert name.encode() in L.spi_last_error() and b"work buffer" in L.spi_last_error()
+        bad = _lib.sim_args(SimParams())
+        bad.decimation = 9
+        assert fn(ctypes.byref(bad), ctypes.byref(cost), fake, 64, 64, fake, 4, 20, 10, fake, ll(need), fake, fake,
+                  None) == 1
+        assert fn(ctypes.byref(args), ctypes.byref(cost), fake, 64, 64, fake, 4, 20, 0, fake, ll(need), fake, fake,
+                  None) == 1
+    ref_cost = _lib.SpiCostArgs(1, (ctypes.c_float * 6)(10, 5, 0, 0, 1, 0))
+    spec = _lib.tangent_spec(list(range(22)))
+    too_many = _lib.tangent_spec(list(range(22)))
+    too_many.n_tan = 32                                        # n_tan + 1 lanes > 32
+    for name in ("spi_residual_jacobian", "spi_residual_jacobian_legs"):
+        fn = getattr(L, name)
+        assert fn(ctypes.byref(args), ctypes.byref(ref_cost), ctypes.byref(spec), fake, 1, 1, fake, fake, 4, 5,
+                  fake, fake, None) == 1
+        assert name.encode() in L.spi_last_error() and b"tracking" in L.spi_last_error()
+        assert fn(ctypes.byref(args), ctypes.byref(cost), ctypes.byref(too_many), fake, 1, 1, fake, fake, 4, 5,
+                  fake, fake, None) == 1
+        assert fn(ctypes.byref(args), ctypes.byref(cost), ctypes.byref(spec), fake, 1, 2, fake, fake, 4, 5,
+                  fake, fake, None) == 1                          # ld < n_cand
