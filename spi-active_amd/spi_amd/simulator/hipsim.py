@@ -22,8 +22,8 @@ import torch
 
 from .. import _lib
 from ..model import ISAAC_BODY_NAMES, load_model
-from ..params import (CONTROL_MODES, MOTOR_MODELS, P_COUNT, P_COMX, P_COMY, P_COMZ, P_IXX, P_IYY, P_IZZ, P_MASS,
-                      SimParams, nominal_params)
+from ..params import (CONTROL_MODES, MOTOR_MODELS, P_COUNT, P_COMX, P_COMZ, P_IXX, P_IXY, P_IXZ, P_IYY,
+                      P_IYZ, P_IZZ, P_MASS, SimParams, nominal_params)
 from ..rollout import RolloutEngine
 from .base import BaseSimulator
 
@@ -214,7 +214,8 @@ class HipSim(BaseSimulator):
         props = []
         for name in self.body_names:
             if name == "base":
-                I = np.array([[p[P_IXX], p[7], p[8]], [p[7], p[P_IYY], p[9]], [p[8], p[9], p[P_IZZ]]])
+                I = np.array([[p[P_IXX], p[P_IXY], p[P_IXZ]], [p[P_IXY], p[P_IYY], p[P_IYZ]],
+                              [p[P_IXZ], p[P_IYZ], p[P_IZZ]]])
                 props.append(RigidBodyProps(p[P_MASS], p[P_COMX:P_COMZ + 1], I))
             else:
                 link = self.model["raw"]["links"][name]
@@ -229,11 +230,41 @@ class HipSim(BaseSimulator):
             if i > 0 and abs(a.mass - b.mass) > 1e-6 * max(abs(b.mass), 1e-3):
                 raise NotImplementedError(f"HipSim identifies the base only; body {self.body_names[i]} changed")
         base = props[0]
-        row = self._params_np[env]
-        row[P_MASS] = base.mass
-        row[P_COMX], row[P_COMY], row[P_COMZ] = base.com.x, base.com.y, base.com.z
-        row[P_IXX], row[P_IYY], row[P_IZZ] = base.inertia.x.x, base.inertia.y.y, base.inertia.z.z
-        self.params[:, env] = torch.as_tensor(row.astype(np.float32), device=self.sim_device)
+        I = base.inertia
+        self.set_body_params([env], "base", mass=base.mass, com=(base.com.x, base.com.y, base.com.z),
+                             inertia=((I.x.x, I.x.y, I.x.z), (I.y.x, I.y.y, I.y.z), (I.z.x, I.z.y, I.z.z)))
+
+    def set_body_params(self, env_ids, name: str, mass=None, com=None, inertia=None) -> None:
+        """Mass, centre of mass and inertia tensor (about the centre of mass, body axes) of body
+        ``name`` for the envs ``env_ids``, in one update of the device parameter array: the
+        native counterpart of the per-env ``gym.get/set_actor_rigid_body_properties`` loop of
+        the plain scripts (scripts/eval.py:185-189, 204-214).  ``mass`` is a scalar or one value
+        per env, ``com`` [3] or [n, 3], ``inertia`` [3, 3] or [n, 3, 3] (symmetric); None leaves a
+        quantity as it is.  The base link is the only body the engine takes as a parameter
+        (the identified one); other bodies keep their URDF values."""
+        if name != "base":
+            raise NotImplementedError(f"HipSim identifies the base only; body {name!r} is fixed")
+        ids = np.asarray(env_ids, dtype=np.int64).reshape(-1)
+        if ids.size == 0:
+            return
+        if ids.min() < 0 or ids.max() >= self.num_envs:
+            raise IndexError("env id out of range")
+        n = ids.size
+        rows = self._params_np[ids]                     # [n, 25] copy
+        if mass is not None:
+            rows[:, P_MASS] = np.broadcast_to(np.asarray(mass, np.float64).reshape(-1), (n,))
+        if com is not None:
+            rows[:, P_COMX:P_COMZ + 1] = np.broadcast_to(np.asarray(com, np.float64).reshape(-1, 3), (n, 3))
+        if inertia is not None:
+            I = np.broadcast_to(np.asarray(inertia, np.float64).reshape(-1, 3, 3), (n, 3, 3))
+            if not np.allclose(I, np.swapaxes(I, 1, 2), rtol=1e-6, atol=1e-12):
+                raise ValueError("inertia tensor must be symmetric")
+            rows[:, P_IXX], rows[:, P_IYY], rows[:, P_IZZ] = I[:, 0, 0], I[:, 1, 1], I[:, 2, 2]
+            rows[:, P_IXY], rows[:, P_IXZ], rows[:, P_IYZ] = I[:, 0, 1], I[:, 0, 2], I[:, 1, 2]
+        self._params_np[ids] = rows
+        dev_ids = torch.as_tensor(ids, device=self.sim_device)
+        self.params[:, dev_ids] = torch.as_tensor(np.ascontiguousarray(rows.T.astype(np.float32)),
+                                                  device=self.sim_device)
 
     # ---------------------------------------------------------------- viewer (headless only)
     def setup_viewer(self):

@@ -101,3 +101,47 @@ def test_eval_script_gym_calls(gpu):
         sim.gym.set_actor_rigid_body_properties(env, actor, ps, recomputeInertia=True)
     sim.gym.refresh_mass_matrix_tensors(sim.sim)
     assert np.allclose(sim.params.cpu().numpy()[0], 7.5)
+
+
+def test_set_body_params_and_gym_shim_agree(gpu):
+    """The native set_body_params (one device update for many envs) writes the base link's mass,
+    centre of mass and full inertia tensor (products included) into the parameter rows, the gym
+    shim reads them back and its write-back is a no-op round trip; the shim also carries products
+    of inertia; other bodies and non-symmetric tensors are refused."""
+    import torch
+    from spi_amd.params import P_COMX, P_COMZ, P_IXX, P_IXY, P_IXZ, P_IYY, P_IYZ, P_IZZ, P_MASS
+    sim, cfg, N = _make_sim()
+    assert N >= 4
+    before = sim.params.clone()
+    ids = [1, 3]
+    mass = np.array([7.1, 8.2])
+    com = np.array([[0.01, -0.02, 0.005], [-0.003, 0.004, 0.02]])
+    I0 = np.array([[0.03, 0.001, -0.002], [0.001, 0.1, 0.0005], [-0.002, 0.0005, 0.11]])
+    I = np.stack([I0, 1.1 * I0])
+    sim.set_body_params(ids, "base", mass=mass, com=com, inertia=I)
+    p = sim.params.cpu().numpy()
+    for k, e in enumerate(ids):
+        assert np.allclose(p[P_MASS, e], mass[k]) and np.allclose(p[P_COMX:P_COMZ + 1, e], com[k])
+        got = [p[r, e] for r in (P_IXX, P_IYY, P_IZZ, P_IXY, P_IXZ, P_IYZ)]
+        want = [I[k, 0, 0], I[k, 1, 1], I[k, 2, 2], I[k, 0, 1], I[k, 0, 2], I[k, 1, 2]]
+        assert np.allclose(got, want, rtol=1e-6)
+    untouched = [e for e in range(N) if e not in ids]
+    assert torch.equal(sim.params[:, untouched], before[:, untouched])
+    props = sim.gym.get_actor_rigid_body_properties(sim.envs[3], sim.robot_handles[3])
+    b = props[0]
+    assert np.isclose(b.mass, 8.2) and np.isclose(b.inertia.x.y, I[1, 0, 1]) and np.isclose(b.inertia.y.z, I[1, 1, 2])
+    snap = sim.params.clone()
+    sim.gym.set_actor_rigid_body_properties(sim.envs[3], sim.robot_handles[3], props, recomputeInertia=True)
+    assert torch.equal(sim.params, snap)
+    props[0].inertia.x.z = props[0].inertia.z.x = 0.004
+    sim.gym.set_actor_rigid_body_properties(sim.envs[3], sim.robot_handles[3], props)
+    assert np.isclose(sim.params[P_IXZ, 3].item(), 0.004)
+    # a scalar for every env at once
+    sim.set_body_params(range(N), "base", mass=6.0)
+    assert torch.allclose(sim.params[P_MASS], torch.full((N,), 6.0, device="cuda:0"))
+    with pytest.raises(NotImplementedError):
+        sim.set_body_params([0], "FL_thigh", mass=1.0)
+    with pytest.raises(ValueError):
+        sim.set_body_params([0], "base", inertia=[[0.1, 0.01, 0.0], [0.0, 0.1, 0.0], [0.0, 0.0, 0.1]])
+    with pytest.raises(IndexError):
+        sim.set_body_params([N], "base", mass=6.0)
