@@ -284,3 +284,39 @@ def test_derivative_entry_points_reject_bad_arguments():
                   fake, fake, None) == 1
         assert fn(ctypes.byref(args), ctypes.byref(cost), ctypes.byref(spec), fake, 1, 2, fake, fake, 4, 5,
                   fake, fake, None) == 1                          # ld < n_cand
+
+
+def test_empty_inputs_return_ok_and_oversized_are_refused():
+    """Zero candidates / segments / samples / excitations return SPI_OK without touching the
+    device (the calls run here, with no GPU); records that do not fit the 64 KB LDS staging
+    (horizon 400 for segments, 1400 for excitations) and CEM updates of more than
+    SPI_MAX_CEM_SAMPLES samples are refused on the host."""
+    import torch  # noqa: F401
+    from spi_amd import _lib
+    from spi_amd.params import SimParams
+    L = _lib.lib()
+    args = _lib.sim_args(SimParams())
+    cost = _lib.SpiCostArgs(0, (ctypes.c_float * 6)(10, 5, 1, 1, 1, 0.05))
+    spec = _lib.tangent_spec(list(range(22)))
+    f = 4096                                                   # never dereferenced
+    a, c, t = ctypes.byref(args), ctypes.byref(cost), ctypes.byref(spec)
+    OK, TOO_LARGE = 0, 3
+    for fn in (L.spi_rollout_cost, L.spi_rollout_cost_legs):
+        assert fn(a, c, f, 0, 0, f, 4, 5, f, None) == OK                       # no candidates
+        assert fn(a, c, f, 8, 8, f, 0, 5, f, None) == OK                       # no segments
+        assert fn(a, c, f, 8, 8, f, 4, 400, f, None) == TOO_LARGE              # record > 64 KB
+    for fn, size in ((L.spi_rollout_cost_adjoint, L.spi_adjoint_work_floats),
+                     (L.spi_rollout_cost_adjoint_legs, L.spi_adjoint_legs_work_floats)):
+        assert fn(a, c, f, 0, 0, f, 4, 20, 10, f, 0, f, f, None) == OK
+        assert fn(a, c, f, 8, 8, f, 4, 400, 10, f, size(8, 4, 400, 10), f, f, None) == TOO_LARGE
+    for fn in (L.spi_traj_sensitivity, L.spi_traj_sensitivity_legs):
+        assert fn(a, t, f, 0, 0, f, f, 4, 50, f, None, f, None) == OK         # no samples
+        assert fn(a, t, f, 8, 8, f, f, 0, 50, f, None, f, None) == OK         # no excitations
+        assert fn(a, t, f, 8, 8, f, f, 4, 1400, f, None, f, None) == TOO_LARGE
+    for fn in (L.spi_residual_jacobian, L.spi_residual_jacobian_legs):
+        assert fn(a, c, t, f, 0, 0, f, f, 4, 5, f, f, None) == OK
+    assert L.spi_rollout_cost_grad(a, c, t, f, 0, 0, f, 4, 5, f, f, None) == OK
+    assert L.spi_rollout_cost_grad(a, c, t, f, 8, 8, f, 4, 400, f, f, None) == TOO_LARGE
+    # the elite sort runs in LDS: at most SPI_MAX_CEM_SAMPLES samples per update
+    assert L.spi_cem_update(f, f, 8193, 22, 8193, 820, 0.0, f, f, f, f, f, None, None) == TOO_LARGE
+    assert b"spi_cem_update" in L.spi_last_error()
