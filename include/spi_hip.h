@@ -423,6 +423,14 @@ int spi_policy_terms(int n_env, int group, const float* states_dev, const float*
                      int step_now, int step_next, float dt_ctrl, const float* q_default_dev, float* gait_dev,
                      const float* applied_dev, float* ring_dev, int ld, int head, int evaluating, float height_lo,
                      float height_hi, hipStream_t stream);
+/* spi_policy_reward and spi_policy_terms in one launch for the ring layout (both read only the
+ * post-step state; the arguments of the two): per control step, one launch fewer. */
+int spi_policy_reward_terms(int n_env, int group, const float* states_dev, const float* commands_dev,
+                            int n_cmd_steps, int step_now, int step_next, float dt_ctrl, const float* q_default_dev,
+                            float* gait_dev, const float* applied_dev, float* ring_dev, int ld, int head,
+                            int evaluating, float height_lo, float height_hi, float delta, float grav_x, float grav_y,
+                            float termination_rew, int* done_dev, const int* bad_dev, float* acc_dev,
+                            int* group_bad_dev, hipStream_t stream);
 int spi_policy_ring_slots(void);
 /* Host table kmap_host[900][2] = (slots back from head, term index) of every observation column. */
 int spi_policy_obs_kmap(int* kmap_host);

@@ -181,12 +181,10 @@ __global__ __launch_bounds__(kPolThreads) void policy_step_leg_kernel(SpiSimArgs
 // terminated group (active_sysid.py:566-568), else ||X||_F^2 with X = (x_main - x_aux) / delta
 // over root state (13) and joint positions (12) of all aux envs (active_sysid_openloop.py:402-426).
 // A non-finite reward or a diverged member marks the group bad.
-__global__ void policy_reward_kernel(const float* __restrict__ s, int n_group, int group, float delta,
-                                     float grav_x, float grav_y, float termination_rew, int* __restrict__ done,
-                                     const int* __restrict__ bad, float* __restrict__ acc,
-                                     int* __restrict__ group_bad) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= n_group) return;
+__device__ __forceinline__ void group_reward(const float* __restrict__ s, int g, int group, float delta, float grav_x,
+                                             float grav_y, float termination_rew, int* __restrict__ done,
+                                             const int* __restrict__ bad, float* __restrict__ acc,
+                                             int* __restrict__ group_bad) {
   const size_t m = static_cast<size_t>(g) * group;
   bool term = false, diverged = false;
   for (int k = 0; k < group; ++k) {
@@ -210,6 +208,15 @@ __global__ void policy_reward_kernel(const float* __restrict__ s, int n_group, i
   }
   if (diverged || !finite_bits(sum)) group_bad[g] = 1;
   acc[g] += term ? termination_rew : sum;
+}
+
+__global__ void policy_reward_kernel(const float* __restrict__ s, int n_group, int group, float delta,
+                                     float grav_x, float grav_y, float termination_rew, int* __restrict__ done,
+                                     const int* __restrict__ bad, float* __restrict__ acc,
+                                     int* __restrict__ group_bad) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_group) return;
+  group_reward(s, g, group, delta, grav_x, grav_y, termination_rew, done, bad, acc, group_bad);
 }
 
 // Gait clock of go2_omni (go2_omni.py:348-379: gait index advanced by dt * frequency, foot
@@ -303,6 +310,28 @@ __global__ __launch_bounds__(kPolThreads) void policy_terms_kernel(
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int g = i / group;
+  float cur[kTerms];
+  obs_terms(s + static_cast<size_t>(i) * SPI_STATE_DIM, cmds + (static_cast<size_t>(g) * T + s_now) * 14,
+            cmds + (static_cast<size_t>(g) * T + s_next) * 14, dt_ctrl, q_default, gait + i,
+            applied + static_cast<size_t>(i) * 12, evaluating, h_lo, h_hi, cur);
+  float* r = ring + static_cast<size_t>(head) * kTerms * ld + i;
+#pragma unroll
+  for (int e = 0; e < kTerms; ++e) r[static_cast<size_t>(e) * ld] = cur[e];
+}
+
+// policy_reward_kernel and policy_terms_kernel in one launch (both read only the post-step
+// state): one lane per env writes its terms; the lane of each group's main env also computes
+// the group's termination and reward.
+__global__ __launch_bounds__(kPolThreads) void policy_reward_terms_kernel(
+    int n, int group, const float* __restrict__ s, const float* __restrict__ cmds, int T, int s_now, int s_next,
+    float dt_ctrl, const float* __restrict__ q_default, float* __restrict__ gait, const float* __restrict__ applied,
+    float* __restrict__ ring, int ld, int head, int evaluating, float h_lo, float h_hi, float delta, float grav_x,
+    float grav_y, float termination_rew, int* __restrict__ done, const int* __restrict__ bad, float* __restrict__ acc,
+    int* __restrict__ group_bad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int g = i / group;
+  if (i % group == 0) group_reward(s, g, group, delta, grav_x, grav_y, termination_rew, done, bad, acc, group_bad);
   float cur[kTerms];
   obs_terms(s + static_cast<size_t>(i) * SPI_STATE_DIM, cmds + (static_cast<size_t>(g) * T + s_now) * 14,
             cmds + (static_cast<size_t>(g) * T + s_next) * 14, dt_ctrl, q_default, gait + i,
@@ -416,6 +445,25 @@ int spi_policy_terms(int n_env, int group, const float* states_dev, const float*
       n_env, group, states_dev, commands_dev, n_cmd_steps, step_now, step_next, dt_ctrl, q_default_dev, gait_dev,
       applied_dev, ring_dev, ld, head, evaluating, height_lo, height_hi);
   return plaunch("policy_terms_kernel");
+}
+
+int spi_policy_reward_terms(int n_env, int group, const float* states_dev, const float* commands_dev,
+                            int n_cmd_steps, int step_now, int step_next, float dt_ctrl, const float* q_default_dev,
+                            float* gait_dev, const float* applied_dev, float* ring_dev, int ld, int head,
+                            int evaluating, float height_lo, float height_hi, float delta, float grav_x, float grav_y,
+                            float termination_rew, int* done_dev, const int* bad_dev, float* acc_dev,
+                            int* group_bad_dev, hipStream_t stream) {
+  if (!groups_ok(n_env, group) || group < 2 || n_cmd_steps < 1 || step_now < 0 || step_next < 0 ||
+      step_now >= n_cmd_steps || step_next >= n_cmd_steps || !(dt_ctrl > 0.0f) || ld < n_env || head < 0 ||
+      head > kHist || !(delta != 0.0f) || !states_dev || !commands_dev || !q_default_dev || !gait_dev ||
+      !applied_dev || !ring_dev || !done_dev || !bad_dev || !acc_dev || !group_bad_dev)
+    return pfail("spi_policy_reward_terms: bad arguments");
+  if (n_env == 0) return SPI_OK;
+  policy_reward_terms_kernel<<<(n_env + kPolThreads - 1) / kPolThreads, kPolThreads, 0, stream>>>(
+      n_env, group, states_dev, commands_dev, n_cmd_steps, step_now, step_next, dt_ctrl, q_default_dev, gait_dev,
+      applied_dev, ring_dev, ld, head, evaluating, height_lo, height_hi, delta, grav_x, grav_y, termination_rew,
+      done_dev, bad_dev, acc_dev, group_bad_dev);
+  return plaunch("policy_reward_terms_kernel");
 }
 
 int spi_policy_ring_slots(void) { return kHist + 1; }

@@ -135,6 +135,8 @@ def lib() -> C.CDLL:
         "spi_mlp_forward": [vp, ip, ip, C.POINTER(ip), vp, vp, C.c_longlong, vp, vp],
         "spi_mlp_forward_ring": [vp, ip, ip, ip, ip, vp, fp, ip, ip, C.POINTER(ip), vp, vp, C.c_longlong, vp, vp],
         "spi_policy_terms": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, ip, ip, ip, fp, fp, vp],
+        "spi_policy_reward_terms": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, ip, ip, ip, fp, fp, fp, fp, fp, fp,
+                                    vp, vp, vp, vp, vp],
         "spi_policy_obs_kmap": [C.POINTER(ip)],
     }
     for name in ("spi_adjoint_work_floats", "spi_adjoint_legs_work_floats"):

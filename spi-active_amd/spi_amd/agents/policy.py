@@ -313,6 +313,23 @@ class PolicyFIMRollout:
                                   self.obs_clip, _lib.stream())
             _lib.check(rc, "spi_policy_obs")
 
+        def reward_observe(p: int, cur: int, now: int) -> None:
+            if not ring:
+                reward(p, cur)
+                observe(p, cur, now)
+                return
+            g0, g1 = parts[p]              # one launch: group rewards + this step's ring slot
+            e0, e1 = g0 * G, g1 * G
+            rg = b["ring"]
+            rc = L.spi_policy_reward_terms(
+                e1 - e0, G, _lib.ptr(s[cur][e0:e1]), _lib.ptr(b["cmds"][g0:g1]), T, now, now + 1, self.dt_ctrl,
+                _lib.ptr(self.q_default), _lib.ptr(b["gait"][e0:e1]), _lib.ptr(b["applied"][e0:e1]),
+                rg.data_ptr() + 4 * e0, rg.shape[2], now % S, self.evaluating, self.height[0], self.height[1],
+                self.delta, self.grav[0], self.grav[1], self.termination_rew, _lib.ptr(b["done"][e0:e1], i32),
+                _lib.ptr(b["bad"][e0:e1], i32), _lib.ptr(b["acc"][g0:g1]), _lib.ptr(b["gbad"][g0:g1], i32),
+                _lib.stream())
+            _lib.check(rc, "spi_policy_reward_terms")
+
         def act(p: int, latest: int) -> None:
             g0, g1 = parts[p]
             e0, e1 = g0 * G, g1 * G
@@ -346,8 +363,7 @@ class PolicyFIMRollout:
                 with torch.cuda.stream(streams[p]):
                     act(p, it - 1)
                     physics(p, cur, pending[p])
-                    reward(p, 1 - cur)
-                    observe(p, 1 - cur, it)
+                    reward_observe(p, 1 - cur, it)
                     pending[p] = sync(p, 1 - cur, it + 1)
             cur = 1 - cur
         for x in streams[1:]:
