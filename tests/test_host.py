@@ -232,6 +232,15 @@ def test_policy_entry_points_reject_bad_arguments():
     # ring head outside the ring
     assert L.spi_policy_terms(8, 2, fake, fake, 10, 1, 2, ctypes.c_float(0.02), fake, fake, fake, fake, 8, 15, 1,
                               ctypes.c_float(-0.25), ctypes.c_float(0.15), None) == 1
+    # the fused reward + ring terms: groups of at least 2, head inside the ring, delta != 0
+    fl = ctypes.c_float
+    def reward_terms(group, head, delta):
+        return L.spi_policy_reward_terms(8, group, fake, fake, 10, 1, 2, fl(0.02), fake, fake, fake, fake, 8, head, 1,
+                                         fl(-0.25), fl(0.15), fl(delta), fl(0.8), fl(0.8), fl(0.0), fake, fake, fake,
+                                         fake, None)
+    assert reward_terms(1, 0, 0.1) == 1 and b"spi_policy_reward_terms" in L.spi_last_error()
+    assert reward_terms(2, 15, 0.1) == 1
+    assert reward_terms(2, 0, 0.0) == 1
     dims = (ctypes.c_int * 3)(70, 33, 12)
     # work buffer too small / ring too narrow
     assert L.spi_mlp_forward(fake, 10, 2, dims, fake, fake, ctypes.c_longlong(10), fake, None) == 1
