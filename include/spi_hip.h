@@ -361,8 +361,9 @@ int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, con
 /* spi_rollout_cost_adjoint with four lanes per rollout (one per leg; base quantities computed
  * in all four, the legs' contributions summed across them): the same outputs up to the order
  * of the legs' sums, a quarter of the registers per lane.  work_dev: at least
- * spi_adjoint_legs_work_floats() floats. */
-long long spi_adjoint_legs_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every);
+ * spi_adjoint_legs_work_floats() floats (block checkpoints and the physics-step states of one
+ * block; decimation = sim->decimation). */
+long long spi_adjoint_legs_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every, int decimation);
 int spi_rollout_cost_adjoint_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
                                   int n_cand, const float* segments_dev, int n_seg, int horizon, int checkpoint_every,
                                   float* work_dev, long long work_floats, float* cost_dev, float* grad_dev,

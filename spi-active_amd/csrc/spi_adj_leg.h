@@ -199,13 +199,11 @@ __device__ __forceinline__ void minv_apply_leg(const LegParams& lp, const LegK& 
 
 // ------------------------------------------------------------------ one sub-interval
 // substep_vjp of spi_adj.h for this lane's leg: reverse of {FD; integrate(h)} at the
-// sub-interval's start state st.  xb: adjoint of the end state on entry, of st on exit.
+// sub-interval's start state st, whose forward dynamics (acc, qdd) the caller kept from the
+// recompute of the sub-interval states.  xb: adjoint of the end state on entry, of st on exit.
 __device__ __forceinline__ void substep_vjp_leg(const SpiSimArgs& a, const LegParams& lp, const LegK& K,
-                                                const Real* raw, const LegState& st, const Real* tau_m, Real h,
-                                                LegStateBar& xb, Real* taub, LegParBar& pb) {
-  M6 acc;
-  Real qdd[3];
-  forward_dynamics_leg(a, lp, K, st, tau_m, acc, qdd);
+                                                const Real* raw, const LegState& st, const M6& acc, const Real* qdd,
+                                                Real h, LegStateBar& xb, Real* taub, LegParBar& pb) {
   const G3 R = quat_to_rot(st.qx, st.qy, st.qz, st.qw);
   const V3 gb{Real(-1.f) * a.gravity * R.m20, Real(-1.f) * a.gravity * R.m21, Real(-1.f) * a.gravity * R.m22};
   const M6 a_rel{acc.w, acc.v - gb};
@@ -382,35 +380,50 @@ struct LegGains {
 };
 
 // ------------------------------------------------------------------ one physics step
+// Sub-interval start states and their forward dynamics, recomputed once into ss / acc / qdd
+// (private memory, indexed by sub-interval), then the sub-interval adjoints backwards.
+struct SubAcc {
+  M6 acc;
+  Real qdd[3];
+};
 __device__ __forceinline__ void physics_step_vjp_leg(const SpiSimArgs& a, const LegParams& lp, const LegK& K,
                                                      const LegGains& G, const Real* raw, const Real* act,
-                                                     const LegState& st0, LegState* ss, LegStateBar& xb,
+                                                     const LegState& st0, LegState* ss, SubAcc* sa, LegStateBar& xb,
                                                      LegParBar& pb) {
   Real tau[3], taub[3];
   control_torques_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, st0.q, st0.qd, tau);
   const Real h = a.dt / Real(a.substeps);
-  ss[0] = st0;
-  for (int k = 1; k < a.substeps; ++k) {
-    ss[k] = ss[k - 1];
+  LegState st = st0;
+  for (int k = 0; k < a.substeps; ++k) {
+    ss[k] = st;
     M6 acc;
     Real qdd[3];
-    forward_dynamics_leg(a, lp, K, ss[k], tau, acc, qdd);
-    integrate_leg(ss[k], acc, qdd, h);
+    forward_dynamics_leg(a, lp, K, st, tau, acc, qdd);
+    sa[k].acc = acc;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) sa[k].qdd[j] = qdd[j];
+    if (k + 1 < a.substeps) integrate_leg(st, acc, qdd, h);
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) taub[k] = Real(0.f);
-  for (int k = a.substeps - 1; k >= 0; --k) substep_vjp_leg(a, lp, K, raw, ss[k], tau, h, xb, taub, pb);
+  for (int k = a.substeps - 1; k >= 0; --k)
+    substep_vjp_leg(a, lp, K, raw, ss[k], sa[k].acc, sa[k].qdd, h, xb, taub, pb);
   control_vjp_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, st0, taub, xb, pb);
 }
 
 // ------------------------------------------------------------------ whole rollout
 // rollout_grad of spi_adj.h, leg-parallel: Store::put / get hold this lane's LegState.  The
-// four lanes of a rollout leave together (the divergence test is the same in all four).
+// forward pass stores the state every `every` control steps (slots 0 .. nb - 1); the reverse
+// pass recomputes each block from its checkpoint and stores the start state of every physics
+// step of the block (slot nb + (t - t0) * decimation + d), so no physics step is recomputed
+// twice.  The four lanes of a rollout leave together (the divergence test is the same in all
+// four).
 template <class Store, class TgtT>
 __device__ bool rollout_grad_leg(const SpiSimArgs& a, const SpiCostArgs& cost, const LegParams& lp, const LegK& K,
                                  const LegGains& G, const Real* raw, const TgtT* rec0, int H, int every, int l,
-                                 Store& store, LegState* ps, LegState* ss, Real& cost_out, LegParBar& pb) {
+                                 Store& store, LegState* ss, SubAcc* sa, Real& cost_out, LegParBar& pb) {
   const int nb = (H + every - 1) / every;
+  const int D = a.decimation;
   LegState st;
   leg_state_from_root(st, rec0, l);
   Real acc(0.f);
@@ -420,7 +433,7 @@ __device__ bool rollout_grad_leg(const SpiSimArgs& a, const SpiCostArgs& cost, c
     Real act[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) act[k] = Real(rec[3 * l + k]);
-    for (int d = 0; d < a.decimation; ++d) {
+    for (int d = 0; d < D; ++d) {
       Real tau[3];
       control_torques_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, st.q, st.qd, tau);
       physics_step_leg(a, lp, K, st, tau);
@@ -438,24 +451,23 @@ __device__ bool rollout_grad_leg(const SpiSimArgs& a, const SpiCostArgs& cost, c
     const int t0 = b * every, t1 = (t0 + every < H) ? t0 + every : H;
     store.get(b, st);
     for (int t = t0; t < t1; ++t) {
-      store.put(nb + (t - t0), st);
-      if (t + 1 < t1) {
-        const TgtT* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
-        Real act[3];
+      const TgtT* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
+      Real act[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) act[k] = Real(rec[3 * l + k]);
-        for (int d = 0; d < a.decimation; ++d) {
-          Real tau[3];
-          control_torques_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, st.q, st.qd, tau);
-          physics_step_leg(a, lp, K, st, tau);
-        }
+      for (int k = 0; k < 3; ++k) act[k] = Real(rec[3 * l + k]);
+      for (int d = 0; d < D; ++d) {
+        store.put(nb + (t - t0) * D + d, st);
+        if (t + 1 == t1 && d + 1 == D) break;       // the block's end state is not needed here
+        Real tau[3];
+        control_torques_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, st.q, st.qd, tau);
+        physics_step_leg(a, lp, K, st, tau);
       }
     }
     for (int t = t1 - 1; t >= t0; --t) {
       const TgtT* rec = rec0 + SPI_STATE_DIM + t * (12 + SPI_STATE_DIM);
       if (cost.mode == SPI_COST_TRACKING || t == H - 1) {
         LegState xe;
-        if (t + 1 < t1) store.get(nb + (t + 1 - t0), xe);
+        if (t + 1 < t1) store.get(nb + (t + 1 - t0) * D, xe);
         else if (b + 1 < nb) store.get(b + 1, xe);
         else xe = final_state;
         step_cost_vjp_leg(cost, xe, rec + 12, l, Real(1.f), xb);
@@ -463,14 +475,11 @@ __device__ bool rollout_grad_leg(const SpiSimArgs& a, const SpiCostArgs& cost, c
       Real act[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) act[k] = Real(rec[3 * l + k]);
-      store.get(nb + (t - t0), ps[0]);
-      for (int d = 1; d < a.decimation; ++d) {
-        ps[d] = ps[d - 1];
-        Real tau[3];
-        control_torques_leg(a, lp, G.kp, G.kd, G.q0, G.effort, act, ps[d].q, ps[d].qd, tau);
-        physics_step_leg(a, lp, K, ps[d], tau);
+      for (int d = D - 1; d >= 0; --d) {
+        LegState ps;
+        store.get(nb + (t - t0) * D + d, ps);
+        physics_step_vjp_leg(a, lp, K, G, raw, act, ps, ss, sa, xb, pb);
       }
-      for (int d = a.decimation - 1; d >= 0; --d) physics_step_vjp_leg(a, lp, K, G, raw, act, ps[d], ss, xb, pb);
     }
   }
   return true;

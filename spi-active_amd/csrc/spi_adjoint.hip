@@ -3,7 +3,7 @@
 // Checkpoints and the recomputed states of the current block live in a caller-provided work
 // buffer laid out [slot][37][lane] so that every put / get is a coalesced access.  The
 // leg-parallel form (spi_adj_leg.h) runs four lanes per rollout with a [slot][19][lane]
-// buffer.
+// buffer holding the block checkpoints and the physics-step states of the current block.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -154,11 +154,12 @@ __global__ __launch_bounds__(kAdjThreads) void cost_adjoint_leg_kernel(
     G.effort[k] = pick4(l, a.effort[k], a.effort[3 + k], a.effort[6 + k], a.effort[9 + k]);
   }
   DevLegStore store{work, static_cast<size_t>(n_seg) * n_cand * 4, rollout * 4 + l};
-  LegState ps[kMaxDecimation], ss[kMaxSubsteps];
+  LegState ss[kMaxSubsteps];
+  SubAcc sa[kMaxSubsteps];
   LegParBar pb;
   zero_par(pb);
   float cst = 0.0f;
-  bool ok = rollout_grad_leg(a, cost, lp, K, G, raw, lds, H, every, l, store, ps, ss, cst, pb);
+  bool ok = rollout_grad_leg(a, cost, lp, K, G, raw, lds, H, every, l, store, ss, sa, cst, pb);
 #pragma unroll
   for (int g = 0; g < 3; ++g) pb.motor[g] = sum4(pb.motor[g]);
   if (ok) {
@@ -236,10 +237,12 @@ int spi_rollout_cost_adjoint(const SpiSimArgs* sim, const SpiCostArgs* cost, con
   return SPI_OK;
 }
 
-long long spi_adjoint_legs_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every) {
-  if (n_cand < 0 || n_seg < 0 || horizon < 1 || checkpoint_every < 1) return -1;
+long long spi_adjoint_legs_work_floats(int n_cand, int n_seg, int horizon, int checkpoint_every, int decimation) {
+  if (n_cand < 0 || n_seg < 0 || horizon < 1 || checkpoint_every < 1 || decimation < 1 || decimation > kMaxDecimation)
+    return -1;
   const long long nb = (horizon + checkpoint_every - 1) / checkpoint_every;
-  return static_cast<long long>(n_cand) * n_seg * 4 * (nb + checkpoint_every) * kLegStateDim;
+  const long long slots = nb + static_cast<long long>(checkpoint_every) * decimation;
+  return static_cast<long long>(n_cand) * n_seg * 4 * slots * kLegStateDim;
 }
 
 int spi_rollout_cost_adjoint_legs(const SpiSimArgs* sim, const SpiCostArgs* cost, const float* params_dev, int ld,
@@ -251,7 +254,7 @@ int spi_rollout_cost_adjoint_legs(const SpiSimArgs* sim, const SpiCostArgs* cost
                                     n_seg, horizon, checkpoint_every, work_dev, cost_dev, grad_dev);
   if (rc != SPI_OK) return rc;
   if (n_cand == 0 || n_seg == 0) return SPI_OK;
-  if (work_floats < spi_adjoint_legs_work_floats(n_cand, n_seg, horizon, checkpoint_every))
+  if (work_floats < spi_adjoint_legs_work_floats(n_cand, n_seg, horizon, checkpoint_every, sim->decimation))
     return fail(SPI_ERR_INVALID_ARG,
                 "spi_rollout_cost_adjoint_legs: work buffer smaller than spi_adjoint_legs_work_floats()");
   const int stride = SPI_STATE_DIM + horizon * (12 + SPI_STATE_DIM);

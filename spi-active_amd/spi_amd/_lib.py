@@ -139,8 +139,9 @@ def lib() -> C.CDLL:
                                     vp, vp, vp, vp, vp],
         "spi_policy_obs_kmap": [C.POINTER(ip)],
     }
+    L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
+    L.spi_adjoint_legs_work_floats.argtypes = [ip, ip, ip, ip, ip]
     for name in ("spi_adjoint_work_floats", "spi_adjoint_legs_work_floats"):
-        getattr(L, name).argtypes = [ip, ip, ip, ip]
         getattr(L, name).restype = C.c_longlong
     L.spi_policy_obs_dim.restype = ip
     L.spi_policy_ring_slots.restype = ip

@@ -151,8 +151,10 @@ def adjoint_cost_grad(engine: RolloutEngine, theta: torch.Tensor, segments: torc
         raise ValueError("segment stride does not match the horizon")
     L = _lib.lib()
     name = "spi_rollout_cost_adjoint_legs" if legs else "spi_rollout_cost_adjoint"
-    sizer = L.spi_adjoint_legs_work_floats if legs else L.spi_adjoint_work_floats
-    wf = int(sizer(n, n_seg, horizon, checkpoint_every))
+    if legs:
+        wf = int(L.spi_adjoint_legs_work_floats(n, n_seg, horizon, checkpoint_every, engine.args.decimation))
+    else:
+        wf = int(L.spi_adjoint_work_floats(n, n_seg, horizon, checkpoint_every))
     if wf < 0:
         raise ValueError("bad sizes")
     dev = theta.device

@@ -265,10 +265,13 @@ def test_derivative_entry_points_reject_bad_arguments():
     fake = vp(4096)
     args = _lib.sim_args(SimParams())
     cost = _lib.SpiCostArgs(0, (ctypes.c_float * 6)(10, 5, 1, 1, 1, 0.05))
-    # four lanes of 19 floats per rollout against one lane of 37
+    # one lane of 37 floats per rollout: checkpoints + one block's control-step states; four
+    # lanes of 19 floats: checkpoints + one block's physics-step states (decimation 4)
     one = L.spi_adjoint_work_floats(64, 4, 20, 10)
-    four = L.spi_adjoint_legs_work_floats(64, 4, 20, 10)
-    assert one == 64 * 4 * (2 + 10) * 37 and four == 64 * 4 * 4 * (2 + 10) * 19
+    four = L.spi_adjoint_legs_work_floats(64, 4, 20, 10, args.decimation)
+    assert args.decimation == 4
+    assert one == 64 * 4 * (2 + 10) * 37 and four == 64 * 4 * 4 * (2 + 10 * 4) * 19
+    assert L.spi_adjoint_legs_work_floats(64, 4, 20, 10, 9) == -1
     for name, need in (("spi_rollout_cost_adjoint", one), ("spi_rollout_cost_adjoint_legs", four)):
         fn = getattr(L, name)
         assert fn(ctypes.byref(args), ctypes.byref(cost), fake, 64, 64, fake, 4, 20, 10, fake, ll(need - 1), fake,
@@ -314,8 +317,8 @@ def test_empty_inputs_return_ok_and_oversized_are_refused():
         assert fn(a, c, f, 0, 0, f, 4, 5, f, None) == OK                       # no candidates
         assert fn(a, c, f, 8, 8, f, 0, 5, f, None) == OK                       # no segments
         assert fn(a, c, f, 8, 8, f, 4, 400, f, None) == TOO_LARGE              # record > 64 KB
-    for fn, size in ((L.spi_rollout_cost_adjoint, L.spi_adjoint_work_floats),
-                     (L.spi_rollout_cost_adjoint_legs, L.spi_adjoint_legs_work_floats)):
+    for fn, size in ((L.spi_rollout_cost_adjoint, lambda *x: L.spi_adjoint_work_floats(*x)),
+                     (L.spi_rollout_cost_adjoint_legs, lambda *x: L.spi_adjoint_legs_work_floats(*x, 4))):
         assert fn(a, c, f, 0, 0, f, 4, 20, 10, f, 0, f, f, None) == OK
         assert fn(a, c, f, 8, 8, f, 4, 400, 10, f, size(8, 4, 400, 10), f, f, None) == TOO_LARGE
     for fn in (L.spi_traj_sensitivity, L.spi_traj_sensitivity_legs):
