@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 
 #include "spi_types.h"
@@ -56,6 +57,63 @@ __global__ __launch_bounds__(256) void fim_assemble_kernel(const float* __restri
     acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
   }
   // result layout: D[i][j], j = lane % 32, i = 8 (r / 4) + 4 (lane / 32) + r % 4
+  float* Fg = F + static_cast<size_t>(g) * P * P;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = 8 * (r >> 2) + 4 * half + (r & 3);
+    if (i < P && col < P) Fg[i * P + col] = acc[r];
+  }
+}
+
+// fim_assemble_kernel with the group's J streamed through LDS: a wavefront reads rows
+// [64 c, 64 c + 64) of its group, a contiguous run of 64 P floats, with coalesced float4 loads
+// (the next chunk in flight while the MFMAs consume this one), stores it to its LDS tile and
+// feeds the MFMAs from there.  Same row pairs in the same order as fim_assemble_kernel: the same
+// result.  Needs K * P % 4 == 0 and 16-byte aligned J; waves past the last group stream the last
+// group and store nothing, so every wave reaches the barriers.
+constexpr int kFimChunk = 64;
+constexpr int kFimPer = kFimChunk * SPI_MAX_TANGENTS / 4 / 64;   // float4 per lane per chunk (P <= 32)
+__global__ __launch_bounds__(256) void fim_assemble_vec_kernel(const float* __restrict__ J, long long G, int K, int P,
+                                                               float* __restrict__ F) {
+  __shared__ float4 tile[4][kFimChunk * SPI_MAX_TANGENTS / 4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long g = static_cast<long long>(blockIdx.x) * 4 + wave;
+  const long long gs = g < G ? g : G - 1;
+  const float4* Jg = reinterpret_cast<const float4*>(J + static_cast<size_t>(gs) * K * P);
+  const int n4 = K * P / 4;                 // float4 of the group
+  const int c4 = kFimChunk * P / 4;         // float4 of a full chunk
+  float4* T4 = tile[wave];
+  const float* T = reinterpret_cast<const float*>(T4);
+  const int col = lane & 31, half = lane >> 5;
+  const bool live = col < P;
+  floatx16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  float4 reg[kFimPer];
+  auto fetch = [&](int c) {
+#pragma unroll
+    for (int i = 0; i < kFimPer; ++i) {
+      const int e = i * 64 + lane, idx = c * c4 + e;
+      reg[i] = (e < c4 && idx < n4) ? Jg[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+  };
+  const int n_chunks = (K + kFimChunk - 1) / kFimChunk;
+  fetch(0);
+  for (int c = 0; c < n_chunks; ++c) {
+#pragma unroll
+    for (int i = 0; i < kFimPer; ++i)
+      if (i * 64 + lane < c4) T4[i * 64 + lane] = reg[i];
+    __syncthreads();
+    if (c + 1 < n_chunks) fetch(c + 1);
+    const int rows = min(kFimChunk, K - c * kFimChunk);
+    for (int k = 0; k < rows; k += 2) {
+      const int kk = k + half;
+      const float v = (live && kk < rows) ? T[kk * P + col] : 0.0f;
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(v, v, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  if (g >= G) return;
   float* Fg = F + static_cast<size_t>(g) * P * P;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -215,6 +273,11 @@ int spi_fim_assemble(const float* J_dev, long long n_groups, int n_rows, int n_t
   if (n_groups == 0) return SPI_OK;
   const long long blocks = (n_groups + 3) / 4;
   if (blocks > 0x7fffffffLL) return ffail(SPI_ERR_TOO_LARGE, "spi_fim_assemble: too many groups");
+  if (static_cast<long long>(n_rows) * n_tan % 4 == 0 && reinterpret_cast<uintptr_t>(J_dev) % 16 == 0 &&
+      static_cast<long long>(n_rows) * n_tan <= 0x7fffffffLL) {
+    fim_assemble_vec_kernel<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(J_dev, n_groups, n_rows, n_tan, F_dev);
+    return flaunch("fim_assemble_vec_kernel");
+  }
   fim_assemble_kernel<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(J_dev, n_groups, n_rows, n_tan, F_dev);
   return flaunch("fim_assemble_kernel");
 }

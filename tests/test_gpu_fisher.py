@@ -94,6 +94,28 @@ def test_fim_assembly_mfma_and_logdet(gpu, sim, theta0):
         obj.scale = None
 
 
+def test_fim_assembly_streamed_equals_rowwise(gpu, sim):
+    """The LDS-streamed assembly (float4 chunks of 64 rows; K P % 4 == 0 and 16-byte aligned J)
+    against the row-pair kernel it replaces, which a J one float off alignment selects: the same
+    row pairs in the same order, so the same bits; odd K, K past one chunk and ragged last
+    chunks, group counts that leave idle wavefronts."""
+    from spi_amd.agents.fisher import FisherObjective
+    from spi_amd.rollout import RolloutEngine
+    obj = FisherObjective(RolloutEngine(sim))
+    rng = np.random.default_rng(5)
+    for G, K, P in ((10, 1250, 22), (7, 131, 4), (5, 64, 32), (3, 740, 23)):
+        Jn = rng.normal(size=(G, K, P)).astype(np.float32)
+        J = torch.tensor(Jn, device="cuda")
+        buf = torch.empty(G * K * P + 1, device="cuda")
+        Jm = buf[1:].view(G, K, P)
+        Jm.copy_(J)
+        assert J.data_ptr() % 16 == 0 and Jm.data_ptr() % 16 != 0
+        F_vec, F_row = obj.assemble(J), obj.assemble(Jm)
+        assert torch.equal(F_vec, F_row), (G, K, P)
+        ref = np.einsum("gki,gkj->gij", Jn.astype(np.float64), Jn.astype(np.float64))
+        assert np.allclose(F_vec.cpu().numpy(), ref, rtol=2e-5, atol=2e-5 * np.abs(ref).max())
+
+
 def test_score_chunking_is_invariant(gpu, sim, theta0):
     from spi_amd.agents.fisher import FisherObjective, fourier_excitations
     from spi_amd.agents.sysid import identification_prior
