@@ -280,6 +280,13 @@ int spi_traj_sensitivity(const SpiSimArgs* sim, const SpiTangentSpec* tan, const
 int spi_traj_sensitivity_legs(const SpiSimArgs* sim, const SpiTangentSpec* tan, const float* params_dev, int ld,
                               int n_samples, const float* init_dev, const float* actions_dev, int n_exc, int horizon,
                               float* J_dev, float* y_dev, int* status_dev, hipStream_t stream);
+/* The same sensitivities with four lanes per (sample, excitation, pair of tangents): each lane
+ * carries the value and two tangents (parameter slots j and j + ceil(n_tan / 2)), so one value
+ * path serves two parameters; about a fifth fewer instructions per tangent than
+ * spi_traj_sensitivity_legs.  Same layout and results up to float rounding. */
+int spi_traj_sensitivity_pairs(const SpiSimArgs* sim, const SpiTangentSpec* tan, const float* params_dev, int ld,
+                               int n_samples, const float* init_dev, const float* actions_dev, int n_exc, int horizon,
+                               float* J_dev, float* y_dev, int* status_dev, hipStream_t stream);
 
 /* Fisher information of every group, F_dev[g] = J_g^T J_g ([n_tan][n_tan], n_tan <= 32),
  * with J_g = J_dev[g] of n_rows x n_tan, accumulated with FP32 MFMA (32x32x2). */

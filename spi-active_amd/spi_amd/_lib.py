@@ -116,6 +116,7 @@ def lib() -> C.CDLL:
         "spi_batch_cem_update": [vp, ip, vp],
         "spi_traj_sensitivity": [sa, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp, vp],
         "spi_traj_sensitivity_legs": [sa, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp, vp],
+        "spi_traj_sensitivity_pairs": [sa, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, vp, ip, ip, vp, vp, vp, vp],
         "spi_fim_assemble": [vp, C.c_longlong, ip, ip, vp, vp],
         "spi_fim_logdet": [vp, vp, ip, ip, ip, vp, C.c_double, vp, vp, vp, vp],
         "spi_rollout_cost_grad": [sa, ca, C.POINTER(SpiTangentSpec), vp, ip, ip, vp, ip, ip, vp, vp, vp],

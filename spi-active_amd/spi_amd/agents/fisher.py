@@ -63,13 +63,17 @@ class FisherObjective:
     """D-optimal excitation scoring from exact sensitivities."""
 
     def __init__(self, engine: RolloutEngine, explore: Sequence[str] = IDENTIFIED, reg: float = 1e-6,
-                 scale: Optional[Sequence[float]] = None, legs: bool = True):
+                 scale: Optional[Sequence[float]] = None, legs: bool = True, pairs: bool = True):
         """scale: per-parameter magnitudes (e.g. the prior mean): the logdet is then that of the
         information about relative parameter changes, comparable across units; None = raw.
-        reg: ridge relative to the mean eigenvalue (trace / P).  legs: tangent rollouts with four
-        lanes per tangent, one per leg (spi_traj_sensitivity_legs)."""
+        reg: ridge relative to the mean eigenvalue (trace / P).  Tangent rollouts: four lanes
+        (one per leg) per two tangents on one value path (spi_traj_sensitivity_pairs, the
+        default: 15 % faster than one tangent per unit), per tangent with pairs=False
+        (spi_traj_sensitivity_legs), or one lane per tangent with legs=False
+        (spi_traj_sensitivity)."""
         self.engine = engine
         self.legs = bool(legs)
+        self.pairs = bool(pairs) and self.legs
         self.names = list(explore)
         self.spec = _lib.tangent_spec(_rows(self.names))
         self.P = len(self.names)
@@ -92,7 +96,10 @@ class FisherObjective:
         y = torch.empty((G, H, FIM_ROWS), dtype=torch.float32, device=dev) if with_y else None
         actions = actions.contiguous()   # keep the (possibly copied) operand alive through the launch
         L = _lib.lib()
-        fn = L.spi_traj_sensitivity_legs if self.legs else L.spi_traj_sensitivity
+        if self.pairs:
+            fn = L.spi_traj_sensitivity_pairs
+        else:
+            fn = L.spi_traj_sensitivity_legs if self.legs else L.spi_traj_sensitivity
         rc = fn(C_ref(self.engine.args), C_ref(self.spec), _lib.ptr(theta), S, S, _lib.ptr(init), _lib.ptr(actions),
                 E, H, _lib.ptr(J), _lib.ptr(y), _lib.ptr(st, torch.int32), _lib.stream())
         _lib.check(rc, "spi_traj_sensitivity")

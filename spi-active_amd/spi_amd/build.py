@@ -19,12 +19,13 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(REPO_ROOT, "include")
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspi_hip.so")
-SOURCES = ["spi_kernels.hip", "spi_cem.hip", "spi_sens.hip", "spi_fisher.hip", "spi_commands.hip",
-           "spi_data.hip", "spi_adjoint.hip", "spi_policy.hip", "spi_mlp.hip"]
+SOURCES = ["spi_kernels.hip", "spi_cem.hip", "spi_sens.hip", "spi_sens_pair.hip", "spi_fisher.hip",
+           "spi_commands.hip", "spi_data.hip", "spi_adjoint.hip", "spi_policy.hip", "spi_mlp.hip"]
 # Per-file flags: the dynamics kernels (primal and tangent) may assume finite arithmetic (x*0 folds away; their NaN
 # and Inf guards test exponent bits); CEM, the cost reduction and the Fisher/logdet kernels
 # keep strict IEEE semantics.
 FILE_FLAGS = {"spi_kernels.hip": ["-ffinite-math-only"], "spi_cem.hip": [], "spi_sens.hip": ["-ffinite-math-only"],
+              "spi_sens_pair.hip": ["-ffinite-math-only"],
               "spi_fisher.hip": [], "spi_commands.hip": [], "spi_data.hip": [],
               "spi_adjoint.hip": ["-ffinite-math-only"], "spi_policy.hip": ["-ffinite-math-only"],
               "spi_mlp.hip": []}
