@@ -189,6 +189,14 @@ template <int M, int N> __device__ __forceinline__ Dual<N> shfl_xor_lane(const D
   for (int i = 0; i < N; ++i) r.d[i] = spi_quad_xor<M>(x.d[i]);
   return r;
 }
+// every component as a leaf for reassociation (spi_quad_pin)
+template <int N> __device__ __forceinline__ Dual<N> quad_pin(const Dual<N>& x) {
+  Dual<N> r;
+  r.v = spi_quad_pin(x.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = spi_quad_pin(x.d[i]);
+  return r;
+}
 
 }  // namespace spi_ad
 

@@ -82,11 +82,14 @@ struct LegParams {
 
 // Sum over the four lanes of an env (lanes 4e .. 4e + 3 of one wavefront, one DPP quad); a
 // scalar type with its own lanes (tangent arithmetic) provides shfl_xor_lane<M> for itself.
-// The butterfly leaves the same bits in the four lanes (float addition commutes).
+// The butterfly leaves the same bits in the four lanes (float addition commutes); its operands
+// and partial sums are pinned so that reassociation cannot regroup them with the code around.
 template <int M> __device__ __forceinline__ float shfl_xor_lane(float x) { return spi_quad_xor<M>(x); }
+__device__ __forceinline__ float quad_pin(float x) { return spi_quad_pin(x); }
 template <class T> __device__ __forceinline__ T sum4(T x) {
-  x += shfl_xor_lane<1>(x);
-  return x + shfl_xor_lane<2>(x);
+  x = quad_pin(x);
+  x = quad_pin(x + shfl_xor_lane<1>(x));
+  return quad_pin(x + shfl_xor_lane<2>(x));
 }
 __device__ __forceinline__ void sum4(V3& v) { v = V3{sum4(v.x), sum4(v.y), sum4(v.z)}; }
 __device__ __forceinline__ void sum4(ABI& I) {

@@ -9,3 +9,11 @@ template <int M> __device__ __forceinline__ float spi_quad_xor(float x) {
   constexpr int kCtrl = M == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] / [2,3,0,1]
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xF, 0xF, false));
 }
+
+// An empty asm the optimiser cannot look through.  A value passed through it is a leaf for
+// reassociation (-fassociative-math), so a quad sum adds the same two operands in each lane
+// (in opposite order, which float addition does not notice) and leaves the same bits in all four.
+__device__ __forceinline__ float spi_quad_pin(float x) {
+  asm("" : "+v"(x));
+  return x;
+}
