@@ -10,10 +10,15 @@ template <int M> __device__ __forceinline__ float spi_quad_xor(float x) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xF, 0xF, false));
 }
 
-// An empty asm the optimiser cannot look through.  A value passed through it is a leaf for
-// reassociation (-fassociative-math), so a quad sum adds the same two operands in each lane
-// (in opposite order, which float addition does not notice) and leaves the same bits in all four.
+// In a source built with -fassociative-math (which the build marks with SPI_REASSOCIATE=1) an
+// empty asm the optimiser cannot look through: a value passed through it is a leaf for
+// reassociation, so a quad sum adds the same two operands in each lane (in opposite order,
+// which float addition does not notice) and leaves the same bits in all four.  Without
+// reassociation the sums cannot be regrouped and the value passes unchanged (the asm would
+// only constrain scheduling).
 __device__ __forceinline__ float spi_quad_pin(float x) {
+#if SPI_REASSOCIATE
   asm("" : "+v"(x));
+#endif
   return x;
 }
