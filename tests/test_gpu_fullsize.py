@@ -3,7 +3,8 @@
 repeated launches are bit-identical, every entry of a full launch equals the same kernel run on
 just that candidate and segment (which checks the grid, block remapping and indexing at full
 size), a sample of entries matches the FP64 oracle, the segment reduction matches a float64 sum,
-and the CEM update of 4096 samples selects numpy's elite set."""
+and the CEM update of 4096 samples selects numpy's elite set.  One chunk of the Fisher workload
+(4096 samples x 16 excitations x H 50, 22 tangents) through the same slice property."""
 import numpy as np
 import pytest
 import torch
@@ -95,3 +96,41 @@ def test_full_size_cem_update_matches_numpy(gpu):
     el = theta[:, order].astype(np.float64)
     assert np.allclose(mean.cpu().numpy(), el.mean(1), atol=1e-5)
     assert np.allclose(std.cpu().numpy(), el.std(1), atol=1e-5)
+
+
+def test_full_size_fisher_chunk_equals_slice(gpu, sim):
+    """One chunk of the Fisher workload at its full size (4096 samples x 16 excitations x H 50,
+    22 tangents on the four-lanes-per-two-tangents kernel: J is 7.2 GB): the J rows and status
+    of chosen (sample, excitation) groups equal the same kernel on just those samples and
+    excitations, bit for bit, and the MFMA assembly of the full chunk equals float64 J^T J of
+    those groups to FP32 accumulation error."""
+    from spi_amd.agents.fisher import FisherObjective, fourier_excitations
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.data import standing_state
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior()
+    rng = np.random.default_rng(3)
+    rows = np.repeat(prior.mean[None], N_SAMPLES, axis=0)
+    rows[:, prior.free] += rng.normal(size=(N_SAMPLES, int(prior.free.sum()))) * prior.std[prior.free] * 0.5
+    rows = np.clip(rows, prior.lo, prior.hi).astype(np.float32)
+    theta = torch.from_numpy(np.ascontiguousarray(rows.T)).to(gpu)
+    n_exc, horizon = 16, 50
+    acts = fourier_excitations(n_exc, horizon, sim.dt * sim.decimation, sim.action_scale, seed=1)
+    init = torch.tensor(standing_state(0.30), dtype=torch.float32, device=gpu)
+    obj = FisherObjective(eng)
+    assert obj.pairs
+    J, st, _ = obj.sensitivities(theta, acts, init)
+    F = obj.assemble(J)
+    cand, exc = [0, 63, 64, 2049, 4095], [0, 7, 15]
+    groups = [e * N_SAMPLES + c for e in exc for c in cand]
+    J_s, st_s, _ = obj.sensitivities(theta[:, cand].contiguous(), acts[exc].contiguous(), init)
+    assert torch.equal(st_s, st[groups])
+    assert torch.equal(J_s, J[groups])
+    Jg = J[groups].double()
+    ref = Jg.transpose(1, 2) @ Jg
+    err = (F[groups].double() - ref).abs().amax(dim=(1, 2))
+    scale = ref.abs().amax(dim=(1, 2))
+    assert torch.all(err <= 2e-4 * scale), (err / scale).max()
+    del J, F
+
