@@ -19,22 +19,23 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 INCLUDE = os.path.join(REPO_ROOT, "include")
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libspi_hip.so")
-SOURCES = ["spi_kernels.hip", "spi_cem.hip", "spi_sens.hip", "spi_sens_pair.hip", "spi_fisher.hip",
+SOURCES = ["spi_kernels.hip", "spi_cem.hip", "spi_sens.hip", "spi_sens_lane.hip", "spi_sens_pair.hip",
+           "spi_fisher.hip",
            "spi_commands.hip", "spi_data.hip", "spi_adjoint.hip", "spi_adjoint_lane.hip", "spi_policy.hip",
            "spi_mlp.hip"]
 # Per-file flags: the dynamics kernels (primal and tangent) may assume finite arithmetic (x*0 folds away; their NaN
-# and Inf guards test exponent bits).  The primal, policy and tangent-pair sources may also
-# reassociate floating-point sums, which lets the compiler turn separate products and sums into
+# and Inf guards test exponent bits).  The primal, policy, four-lane tangent, tangent-pair and
+# four-lane adjoint sources may also reassociate floating-point sums, which lets the compiler turn separate products and sums into
 # fused multiply-adds (primal: 3,748 -> 3,454 instructions per articulated-body sub-interval,
 # tangent pairs: 6,718 -> 5,867 per kernel); the four-lane sums pin their operands so that
-# reassociation cannot make them lane-dependent.  The one-lane adjoint (spi_adjoint_lane.hip)
-# and the tangent kernels of spi_sens.hip (one lane per tangent, and the four-lane forms that
-# are no longer the defaults) do without it: the one-lane kernels already spill, and
+# reassociation cannot make them lane-dependent.  The one-lane adjoint and tangent kernels
+# (spi_adjoint_lane.hip, spi_sens_lane.hip) do without it: they already spill, and
 # reassociation keeps more values live (one-lane adjoint 461 -> 540 ms).
 # CEM, the cost reduction and the Fisher/logdet kernels keep strict IEEE semantics.
 DYN_FLAGS = ["-ffinite-math-only"]
 DYN_REASSOC_FLAGS = DYN_FLAGS + ["-fassociative-math", "-DSPI_REASSOCIATE=1"]
-FILE_FLAGS = {"spi_kernels.hip": DYN_REASSOC_FLAGS, "spi_cem.hip": [], "spi_sens.hip": DYN_FLAGS,
+FILE_FLAGS = {"spi_kernels.hip": DYN_REASSOC_FLAGS, "spi_cem.hip": [], "spi_sens.hip": DYN_REASSOC_FLAGS,
+              "spi_sens_lane.hip": DYN_FLAGS,
               "spi_sens_pair.hip": DYN_REASSOC_FLAGS,
               "spi_fisher.hip": [], "spi_commands.hip": [], "spi_data.hip": [],
               "spi_adjoint.hip": DYN_REASSOC_FLAGS, "spi_adjoint_lane.hip": DYN_FLAGS,
