@@ -72,13 +72,28 @@ __global__ void cem_sample_kernel(const float* __restrict__ mean, const float* _
 
 // cost_dev[c] = sum_s w[s] * cost_seg[s][c]: one thread per candidate, segments in a fixed
 // order, so the per-candidate cost (and the CEM ranking) is deterministic.  Built with strict
-// IEEE semantics so +inf (diverged) costs propagate.
+// IEEE semantics so +inf (diverged) costs propagate; a segment of weight 0 is skipped (its
+// cost may be +inf).  The loads of eight segments are issued together (they are independent;
+// one at a time, a 256-candidate landscape waited on 64 dependent load latencies), the sum
+// keeps the segment order.
 __global__ void reduce_segments_kernel(const float* __restrict__ cs, const float* __restrict__ w, int n_seg,
                                        int n_cand, float* __restrict__ out) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n_cand) return;
   float acc = 0.0f;
-  for (int s = 0; s < n_seg; ++s) {
+  int s = 0;
+  for (; s + 8 <= n_seg; s += 8) {
+    float ws[8], v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      ws[k] = w[s + k];
+      v[k] = cs[static_cast<size_t>(s + k) * n_cand + c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (ws[k] != 0.0f) acc += ws[k] * v[k];
+  }
+  for (; s < n_seg; ++s) {
     float ws = w[s];
     if (ws != 0.0f) acc += ws * cs[static_cast<size_t>(s) * n_cand + c];
   }

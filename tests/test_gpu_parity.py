@@ -338,6 +338,29 @@ def test_cem_sample_matches_oracle_stream(gpu):
     assert np.abs(theta.cpu().numpy() - ref).max() <= 2e-5
 
 
+@pytest.mark.parametrize("n_seg,n_cand", [(64, 256), (13, 300), (7, 5)])
+def test_segment_reduction_matches_float64(gpu, sim, n_seg, n_cand):
+    """cost[c] = sum_s w_s cost[s][c] in segment order: float64 sum to FP32 rounding, segments of
+    weight 0 skipped even where their cost is +inf, +inf propagated from weighted segments; the
+    segment counts cover whole groups of eight loads and a remainder."""
+    import torch
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    rng = np.random.default_rng(n_seg)
+    cs = rng.gamma(2.0, 1.0, (n_seg, n_cand)).astype(np.float32)
+    w = rng.uniform(0.0, 1.0, n_seg).astype(np.float32)
+    w[::3] = 0.0
+    cs[0, :] = np.inf                               # weight 0: must not reach the sum
+    cs[1, 1] = np.inf                               # weighted: candidate 1 is +inf
+    got = eng.reduce(torch.from_numpy(cs).cuda(), torch.from_numpy(w).cuda()).cpu().numpy()
+    keep = w != 0.0
+    ref = (w[keep, None].astype(np.float64) * cs[keep].astype(np.float64)).sum(0)
+    assert np.isinf(got[1]) and np.isinf(ref[1])
+    fin = np.isfinite(ref)
+    assert np.all(np.isfinite(got[fin]))
+    assert np.allclose(got[fin], ref[fin], rtol=2e-6, atol=0.0)
+
+
 def test_cem_update_matches_numpy(gpu):
     """Elite set (sorted by cost, ties by index, NaN last) identical; moments within 1e-5."""
     import torch
