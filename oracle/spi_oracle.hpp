@@ -26,6 +26,18 @@
 #include <vector>
 #include <algorithm>
 
+// SPIO_MUTATION plants one known mistake in the time stepping, so that the closed-form
+// integrator tests can show that each of them fails (built only by `make mutants` into
+// lib/libspi_oracle_mut<k>.so; no parity test loads these):
+//   1  explicit Euler: positions advanced with the velocities from before the update
+//   2  world-frame angular velocity in the right-multiplied quaternion increment
+//   3  body-frame angular velocity in a left-multiplied (world-side) increment
+//   4  sub-interval length dt instead of dt / substeps
+//   5  torque computed once per control step instead of once per physics step
+#ifndef SPIO_MUTATION
+#define SPIO_MUTATION 0
+#endif
+
 namespace spio {
 
 using std::size_t;
@@ -645,29 +657,54 @@ template <class T> struct Engine {
 
   // Semi-implicit Euler: velocities first, then positions with the new velocities.
   void integrate(State<T>& st, const V6<T>& a0, const T* qdd) const {
+#if SPIO_MUTATION == 4
+    T dt = T(cfg.dt);
+#else
     T dt = T(cfg.dt) / T(cfg.substeps);
+#endif
     V6<T> v0 = base_body_velocity(st);
+#if SPIO_MUTATION == 1
+    const V6<T> v_old = v0;
+#endif
     for (int d = 0; d < 6; ++d) v0[d] += dt * a0[d];
     for (int j = 0; j < 12; ++j) {
+#if SPIO_MUTATION == 1
+      st.q[j] += dt * st.qd[j];
+      st.qd[j] += dt * qdd[j];
+#else
       st.qd[j] += dt * qdd[j];
       st.q[j] += dt * st.qd[j];
+#endif
     }
     M3<T> R = quat_matrix(st.root + 3);
     V3<T> vb{v0[3], v0[4], v0[5]}, wb{v0[0], v0[1], v0[2]};
+#if SPIO_MUTATION == 1
+    V3<T> vw = mv3(R, V3<T>{v_old[3], v_old[4], v_old[5]});
+#else
     V3<T> vw = mv3(R, vb);
+#endif
     for (int d = 0; d < 3; ++d) st.root[d] += dt * vw[d];
     // q <- q (x) exp(w dt / 2), body-frame angular velocity
-    T th2 = (wb[0] * wb[0] + wb[1] * wb[1] + wb[2] * wb[2]) * dt * dt;
+#if SPIO_MUTATION == 2
+    V3<T> wq = mv3(R, wb);
+#else
+    V3<T> wq = wb;
+#endif
+    T th2 = (wq[0] * wq[0] + wq[1] * wq[1] + wq[2] * wq[2]) * dt * dt;
     T dq[4];
     if (s_pos(th2)) {
       T th = s_sqrt(th2);
       T s = s_sin(th * T(0.5)) / th;
-      dq[0] = wb[0] * dt * s; dq[1] = wb[1] * dt * s; dq[2] = wb[2] * dt * s; dq[3] = s_cos(th * T(0.5));
+      dq[0] = wq[0] * dt * s; dq[1] = wq[1] * dt * s; dq[2] = wq[2] * dt * s; dq[3] = s_cos(th * T(0.5));
     } else {
       dq[0] = dq[1] = dq[2] = T(0); dq[3] = T(1);
     }
     T qn[4];
+#if SPIO_MUTATION == 3
+    quat_mul(dq, st.root + 3, qn);
+#else
     quat_mul(st.root + 3, dq, qn);
+#endif
     T nrm = s_sqrt(qn[0] * qn[0] + qn[1] * qn[1] + qn[2] * qn[2] + qn[3] * qn[3]);
     for (int d = 0; d < 4; ++d) st.root[3 + d] = qn[d] / nrm;
     M3<T> Rn = quat_matrix(st.root + 3);
@@ -687,11 +724,17 @@ template <class T> struct Engine {
   // One control step: torques recomputed at every physics step from the current state
   // (legged_robot_base.py:201-209).
   void control_step(State<T>& st, const T* action, const T* par) const {
+#if SPIO_MUTATION == 5
+    T tau[12];
+    control_torque(cfg, action, st.q, st.qd, par, tau);
+    for (int s = 0; s < cfg.decimation; ++s) physics_step(st, tau, par);
+#else
     for (int s = 0; s < cfg.decimation; ++s) {
       T tau[12];
       control_torque(cfg, action, st.q, st.qd, par, tau);
       physics_step(st, tau, par);
     }
+#endif
   }
 };
 

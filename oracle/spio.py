@@ -46,14 +46,21 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        build()
-        L = C.CDLL(LIB_PATH)
+def mutant_path(k: int) -> str:
+    """Oracle build with the planted integrator mistake k (SPIO_MUTATION in spi_oracle.hpp)."""
+    return os.path.join(HERE, "lib", f"libspi_oracle_mut{k}.so")
+
+
+def lib(path: Optional[str] = None):
+    """The oracle library (default build, or the one at `path`, e.g. a mutant)."""
+    key = path or LIB_PATH
+    if key not in _libs:
+        if path is None:
+            build()
+        L = C.CDLL(key)
         dp = C.POINTER(C.c_double)
         fp = C.POINTER(C.c_float)
         vp = C.c_void_p
@@ -87,8 +94,8 @@ def lib():
         assert L.spio_sizeof_config() == C.sizeof(SimConfig)
         assert L.spio_sizeof_link() == C.sizeof(LinkDesc)
         assert L.spio_sizeof_contact() == C.sizeof(ContactDesc)
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
 
 
 def _dp(a: np.ndarray):
@@ -195,8 +202,9 @@ def make_links(specs):
 class Oracle:
     """FP64 oracle engine for one configuration."""
 
-    def __init__(self, model: Optional[Dict], sim, f32: bool = False, links=None, contacts=None):
-        L = lib()
+    def __init__(self, model: Optional[Dict], sim, f32: bool = False, links=None, contacts=None,
+                 lib_path: Optional[str] = None):
+        L = self._L = lib(lib_path)
         if model is not None:
             self.links, self.contacts, self.order = unfolded_tree(model, model["dof_names"])
         else:
@@ -212,7 +220,7 @@ class Oracle:
     def __del__(self):
         try:
             if self.h:
-                (lib().spio_free_f32 if self.f32 else lib().spio_free)(self.h)
+                (self._L.spio_free_f32 if self.f32 else self._L.spio_free)(self.h)
         except Exception:
             pass
 
@@ -220,43 +228,43 @@ class Oracle:
     def forward(self, root, q, qd, tau, par):
         root, q, qd, tau, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (root, q, qd, tau, par))
         a0, qdd, ff, ta = np.zeros(6), np.zeros(12), np.zeros(12), np.zeros(12)
-        lib().spio_forward(self.h, _dp(root), _dp(q), _dp(qd), _dp(tau), _dp(par), _dp(a0), _dp(qdd), _dp(ff), _dp(ta))
+        self._L.spio_forward(self.h, _dp(root), _dp(q), _dp(qd), _dp(tau), _dp(par), _dp(a0), _dp(qdd), _dp(ff), _dp(ta))
         return a0, qdd, ff.reshape(4, 3), ta
 
     def rnea(self, root, q, qd, a0, qdd, par):
         root, q, qd, a0, qdd, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (root, q, qd, a0, qdd, par))
         tb, t = np.zeros(6), np.zeros(12)
-        lib().spio_rnea(self.h, _dp(root), _dp(q), _dp(qd), _dp(a0), _dp(qdd), _dp(par), _dp(tb), _dp(t))
+        self._L.spio_rnea(self.h, _dp(root), _dp(q), _dp(qd), _dp(a0), _dp(qdd), _dp(par), _dp(tb), _dp(t))
         return tb, t
 
     def crba(self, root, q, par):
         root, q, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (root, q, par))
         H = np.zeros((18, 18))
-        lib().spio_crba(self.h, _dp(root), _dp(q), _dp(par), _dp(H))
+        self._L.spio_crba(self.h, _dp(root), _dp(q), _dp(par), _dp(H))
         return H
 
     def momentum(self, root, q, qd, par):
         root, q, qd, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (root, q, qd, par))
         h, e = np.zeros(6), C.c_double(0.0)
-        lib().spio_momentum(self.h, _dp(root), _dp(q), _dp(qd), _dp(par), _dp(h), C.byref(e))
+        self._L.spio_momentum(self.h, _dp(root), _dp(q), _dp(qd), _dp(par), _dp(h), C.byref(e))
         return h, e.value
 
     def fk(self, root, q):
         root, q = (np.ascontiguousarray(x, dtype=np.float64) for x in (root, q))
         lp, fp = np.zeros((len(self.links), 3)), np.zeros((len(self.contacts), 3))
-        lib().spio_fk(self.h, _dp(root), _dp(q), _dp(lp), _dp(fp))
+        self._L.spio_fk(self.h, _dp(root), _dp(q), _dp(lp), _dp(fp))
         return lp, fp
 
     def physics_step(self, root, q, qd, tau, par):
         root, q, qd = (np.array(x, dtype=np.float64) for x in (root, q, qd))
         tau, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (tau, par))
-        lib().spio_physics_step(self.h, _dp(root), _dp(q), _dp(qd), _dp(tau), _dp(par))
+        self._L.spio_physics_step(self.h, _dp(root), _dp(q), _dp(qd), _dp(tau), _dp(par))
         return root, q, qd
 
     def control_torque(self, action, q, qd, par):
         action, q, qd, par = (np.ascontiguousarray(x, dtype=np.float64) for x in (action, q, qd, par))
         tau = np.zeros(12)
-        lib().spio_control_torque(C.byref(self.cfg), _dp(action), _dp(q), _dp(qd), _dp(par), _dp(tau))
+        self._L.spio_control_torque(C.byref(self.cfg), _dp(action), _dp(q), _dp(qd), _dp(par), _dp(tau))
         return tau
 
     def simulate(self, par, init37, actions):
@@ -265,7 +273,7 @@ class Oracle:
         actions = np.ascontiguousarray(actions, dtype=np.float32)
         T = actions.shape[0]
         out = np.zeros((T, 37), dtype=np.float32)
-        rc = lib().spio_simulate(self.h, _dp(par), _fp(init37), _fp(actions), T, _fp(out))
+        rc = self._L.spio_simulate(self.h, _dp(par), _fp(init37), _fp(actions), T, _fp(out))
         return out, rc
 
     def rollout_cost(self, par, seg_init, seg_actions, seg_targets, cost_mode, weights):
@@ -276,7 +284,7 @@ class Oracle:
         w = np.ascontiguousarray(weights, dtype=np.float64)
         n_seg, H = seg_actions.shape[0], seg_actions.shape[1]
         out = np.zeros((par.shape[0], n_seg))
-        lib().spio_rollout_cost(self.h, _dp(par), par.shape[0], _fp(seg_init), _fp(seg_actions), _fp(seg_targets),
+        self._L.spio_rollout_cost(self.h, _dp(par), par.shape[0], _fp(seg_init), _fp(seg_actions), _fp(seg_targets),
                                 n_seg, H, cost_mode, _dp(w), _dp(out))
         return out
 
@@ -289,7 +297,7 @@ class Oracle:
         actions = np.ascontiguousarray(actions, dtype=np.float32)
         T, P = actions.shape[0], idx.shape[0]
         y, J = np.zeros((T, n_rows)), np.zeros((T, n_rows, P))
-        rc = lib().spio_traj_sens(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), P, _fp(init37),
+        rc = self._L.spio_traj_sens(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), P, _fp(init37),
                                   _fp(actions), T, n_rows, _dp(y), _dp(J))
         return y, J, rc
 
@@ -303,7 +311,7 @@ class Oracle:
         w = np.ascontiguousarray(weights, dtype=np.float64)
         n_seg, H = seg_actions.shape[0], seg_actions.shape[1]
         cost, grad = np.zeros(n_seg), np.zeros((n_seg, idx.shape[0]))
-        lib().spio_cost_grad(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), idx.shape[0], _fp(seg_init),
+        self._L.spio_cost_grad(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), idx.shape[0], _fp(seg_init),
                              _fp(seg_actions), _fp(seg_targets), n_seg, H, cost_mode, _dp(w), _dp(cost), _dp(grad))
         return cost, grad
 
@@ -316,7 +324,7 @@ class Oracle:
         w = np.ascontiguousarray(weights, dtype=np.float64)
         n_seg, H = seg_actions.shape[0], seg_actions.shape[1]
         out = np.zeros((par.shape[0], n_seg), dtype=np.float32)
-        lib().spio_rollout_cost_f32(self.h, _fp(par), par.shape[0], _fp(seg_init), _fp(seg_actions),
+        self._L.spio_rollout_cost_f32(self.h, _fp(par), par.shape[0], _fp(seg_init), _fp(seg_actions),
                                     _fp(seg_targets), n_seg, H, cost_mode, _dp(w), _fp(out), n_threads)
         return out
 
