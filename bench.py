@@ -251,26 +251,82 @@ def plain_gpu_baseline(model, sim, segs, prior, weights):
             "sample": "1 control step (4 physics steps x 4 sub-intervals) per configuration"}
 
 
-def cpu_baseline(model, sim, segs, prior, n_cand):
-    """Oracle FP32 engine (OpenMP over the host cores) on a bounded sample of the workload."""
+def host_cores():
+    """Cores this process may use: the host's CPU share as the environment sets it for OpenMP
+    (OMP_NUM_THREADS), else every core of the affinity mask; and the size of that mask."""
+    visible = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "").strip()
+    return (int(env) if env.isdigit() and int(env) > 0 else visible), visible
+
+
+def _rows_from_prior(prior, n, seed=0):
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(prior.mean[None], n, axis=0)
+    rows[:, prior.free] += rng.normal(size=(n, int(prior.free.sum()))) * prior.std[prior.free]
+    return np.clip(rows, prior.lo, prior.hi).astype(np.float32)
+
+
+def cpu_baseline(sim, seg_packed, prior, weights, target_s=10.0):
+    """The GPU kernel's algorithm on the host: spi_dyn.h built for x86-64-v3 with OpenMP
+    (tests/host_dyn/spi_rollout_cpu.cpp), over all the cores this process may use, on a bounded
+    sample of the workload's candidates (all 256 segments x H 20, about target_s seconds)."""
+    import ctypes
+    try:
+        from spi_amd import _lib
+        lib_path = os.path.join(ROOT, "tests", "host_dyn", "lib", "libspi_rollout_cpu.so")
+        if not os.path.exists(lib_path):
+            import subprocess
+            subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "host_dyn"), "-s"], check=True)
+        L = ctypes.CDLL(lib_path)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.spi_cpu_rollout_cost.argtypes = [ctypes.c_void_p, ctypes.c_void_p, fp, ctypes.c_int, ctypes.c_int, fp,
+                                           ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
+        cores, visible = host_cores()
+        a, c = _lib.sim_args(sim), _lib.cost_args(0, weights)
+        seg = np.ascontiguousarray(seg_packed, np.float32)
+        n_seg = seg.shape[0]
+
+        def run(n):
+            par = np.ascontiguousarray(_rows_from_prior(prior, n).T)
+            out = np.zeros((n_seg, n), np.float32)
+            t = time.perf_counter()
+            used = L.spi_cpu_rollout_cost(ctypes.byref(a), ctypes.byref(c), par.ctypes.data_as(fp), n, n,
+                                          seg.ctypes.data_as(fp), n_seg, HORIZON, out.ctypes.data_as(fp), cores)
+            return time.perf_counter() - t, used
+
+        dt, _ = run(16)
+        n = int(min(N_SAMPLES, max(16, 16 * target_s / max(dt, 1e-3))))
+        dt, used = run(n)
+        steps = n * n_seg * HORIZON
+        return {"value": round(steps / dt, 1), "unit": "env-steps/s", "cores": used, "kind": "port",
+                "algorithm": "same as the GPU kernel: spi_dyn.h built for the host (x86-64-v3, OpenMP), one "
+                             "rollout per thread iteration",
+                "cores_visible": visible,
+                "sample": f"{n} of {N_SAMPLES} CEM candidates x {n_seg} segments x H={HORIZON} "
+                          f"({steps} control steps, {dt:.1f} s)"}
+    except Exception as e:  # the baseline is reported, never fatal
+        return {"value": None, "unit": "env-steps/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}
+
+
+def cpu_baseline_oracle(model, sim, segs_np, prior, n_cand):
+    """The oracle's FP32 engine (generic dense spatial algebra on the unfolded 35-link URDF
+    tree, OpenMP over the same cores) on a bounded sample of the workload."""
     try:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import spio
         spio.build()
-        cores = len(os.sched_getaffinity(0))
-        cores = min(cores, 16)
+        cores, visible = host_cores()
         o = spio.Oracle(model, sim, f32=True)
-        rng = np.random.default_rng(0)
-        rows = np.repeat(prior.mean[None], n_cand, axis=0)
-        rows[:, prior.free] += rng.normal(size=(n_cand, int(prior.free.sum()))) * prior.std[prior.free]
-        rows = np.clip(rows, prior.lo, prior.hi).astype(np.float32)
+        rows = _rows_from_prior(prior, n_cand)
         w = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
         t = time.perf_counter()
-        o.rollout_cost_f32(rows, segs.init, segs.actions, segs.targets, 0, w, n_threads=cores)
+        o.rollout_cost_f32(rows, segs_np["init"], segs_np["actions"], segs_np["targets"], 0, w, n_threads=cores)
         dt = time.perf_counter() - t
-        steps = n_cand * segs.n_seg * segs.horizon
+        n_seg = segs_np["init"].shape[0]
+        steps = n_cand * n_seg * HORIZON
         return {"value": round(steps / dt, 1), "unit": "env-steps/s", "cores": cores, "kind": "port",
-                "sample": f"{n_cand} of {N_SAMPLES} CEM candidates x {segs.n_seg} segments x H={segs.horizon} "
+                "cores_visible": visible,
+                "sample": f"{n_cand} of {N_SAMPLES} CEM candidates x {n_seg} segments x H={HORIZON} "
                           f"({steps} control steps, {dt:.1f} s), FP32 oracle engine on the unfolded URDF tree"}
     except Exception as e:  # the baseline is reported, never fatal
         return {"value": None, "unit": "env-steps/s", "cores": 0, "kind": "port", "sample": f"failed: {e}"}

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define SPI_ABI_VERSION 1
+#define SPI_ABI_VERSION 2
 #define SPI_NUM_DOF 12
 #define SPI_ROOT_DIM 13
 #define SPI_STATE_DIM 37 /* root(13) + q(12) + qd(12) */
@@ -77,6 +77,11 @@ typedef struct SpiSimArgs {
 typedef struct SpiCostArgs {
   int mode;            /* SpiCostMode */
   float w[6];          /* weights: pos, quat, linvel, angvel, q, qd (reference mode uses 0, 1, 4) */
+  /* Recorded PD gains, device [n_seg][2] = (kp, kd) per segment, or NULL.  When set, a rollout
+   * of segment s runs its PD law with kp = seg_gains[2s] and kd = seg_gains[2s+1] on all 12
+   * joints in place of SpiSimArgs kp/kd, as scripts/eval.py:192-201 (apply_pd_gains_batch)
+   * sets env.p_gains/d_gains from the recorded pd_gain_kp/kd of each chunk (:245-250). */
+  const float* seg_gains;
 } SpiCostArgs;
 
 /* One problem of a batched CEM launch (device-resident array of these).  Every pointer is
@@ -214,14 +219,38 @@ int spi_simulate(const SpiSimArgs* sim, const float* params_dev, int ld, int n, 
  * parameter), all envs of a group replay actions_dev [n_group][T][12] from their own initial
  * state init_dev [n_env][37].  After every control step the group's FD-FIM trace
  * ||X||_F^2 (X = (main - aux) / delta over root(13) + q(12)) is accumulated;
- * fim_dev [n_group] receives the sum over the T steps.  Every ksync steps (at steps
- * t % ksync == 1, counting from 1) the aux envs are synchronised to the main env:
+ * fim_dev [n_group] receives the sum over the T steps.  The T steps are the command steps
+ * that follow reset_all's zero-action step (base_task.py:90-100), which the reference counts
+ * as step_idx 1: command step t (0-based) has step_idx t + 2, and after it the aux envs are
+ * synchronised to the main env when step_idx % ksync == 1, or after every step for
+ * ksync == 1 (active_sysid_openloop.py:246-252); with ksync 5 that is after command steps
+ * 5, 10, 15 (1-based).  init_dev holds the states after the reset step (and its sync).
  * ksync_mode 0 = as the reference does it (only the first physics step's PD torque sees the
  * main's joint state, active_sysid_openloop.py:283-290 writes views that are not pushed),
  * 1 = full state copy.  ksync <= 0 disables.  n_param + 1 <= 64. */
 int spi_fim_rollout(const SpiSimArgs* sim, const float* params_dev, int ld, int n_group, int n_param,
                     const float* init_dev, const float* actions_dev, int n_steps, float delta, int ksync,
                     int ksync_mode, float* fim_dev, hipStream_t stream);
+
+/* ---- env reset (active_sysid_openloop.py:116-162, 274-292; legged_robot_base.py:761-784) ---- */
+
+/* What a reset draws around: q = q_default * U[q_lo, q_hi] per joint, qd = 0 (_reset_dofs,
+ * active_sysid_openloop.py:274-292); base at root0 = (pos xyz, quat xyzw) of the init state
+ * (go2.yaml init_state, plus the env origin), linear and angular velocity U[-vel_range, vel_range]
+ * each (_reset_root_states, legged_robot_base.py:761-784). */
+typedef struct SpiResetSpec {
+  float q_default[12];
+  float root0[7];
+  float q_lo, q_hi;       /* 0.5, 1.5 in the reference */
+  float vel_range;        /* 0.5 in the reference */
+} SpiResetSpec;
+
+/* The reset states of n_env envs drawn on the device, out_dev [n_env][37] = root(13) + q(12) +
+ * qd(12).  Uniforms from Philox4x32-10 under key `seed`, counter (env, block, 'RSET'): env e's
+ * draw depends only on (seed, e).  origins_dev [n_env][3] (env origins added to the base
+ * position) or NULL.  Replaces the torch_rand_float draws of reset_all (base_task.py:90-100). */
+int spi_reset_states(const SpiResetSpec* spec, int n_env, const float* origins_dev, unsigned long long seed,
+                     float* out_dev, hipStream_t stream);
 
 /* ---- CEM (replaces the Optuna sampler of algo/active_sysid.yaml:20 / mass_opt.py:216) ---- */
 

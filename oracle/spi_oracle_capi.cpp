@@ -297,6 +297,66 @@ int spio_traj_sens(void* hp, const double* par, const int* idx, int P, const flo
   return finite_state(st) ? 0 : 1;
 }
 
+// spio_traj_sens over n_traj trajectories in parallel (OpenMP; n_threads <= 0: the runtime's
+// default): init37 [n_traj][37], actions [n_traj][n_steps][12], y_out [n_traj][n_steps][n_rows],
+// J_out [n_traj][n_steps][n_rows][P].  Returns the number of trajectories that diverged.
+int spio_traj_sens_batch(void* hp, const double* par, const int* idx, int P, const float* init37,
+                         const float* actions, int n_traj, int n_steps, int n_rows, double* y_out, double* J_out,
+                         int n_threads) {
+  if (P < 1 || P > P_COUNT || n_rows < 1 || n_rows > 37 || n_traj < 0) return -1;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+  int bad = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : bad)
+  for (int i = 0; i < n_traj; ++i) {
+    const size_t ys = size_t(n_steps) * n_rows;
+    bad += spio_traj_sens(hp, par, idx, P, init37 + size_t(i) * 37, actions + size_t(i) * n_steps * 12, n_steps,
+                          n_rows, y_out + i * ys, J_out + i * ys * P) != 0;
+  }
+  return bad;
+}
+
+// FP64 rollout cost of n_cand parameter vectors on n_seg segments with the (candidate, segment)
+// pairs spread over OpenMP threads; same arguments and result as spio_rollout_cost.
+int spio_rollout_cost_par(void* hp, const double* par, int n_cand, const float* seg_init, const float* seg_actions,
+                          const float* seg_targets, int n_seg, int H, int cost_mode, const double* w,
+                          double* cost_out, int n_threads) {
+  auto* h = static_cast<Handle<double>*>(hp);
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+  int bad = 0;
+  const int total = n_cand * n_seg;
+#pragma omp parallel reduction(+ : bad)
+  {
+    Engine<double> eng = h->eng;  // per-thread copy (base inertia is overwritten per candidate)
+#pragma omp for schedule(dynamic, 2)
+    for (int lane = 0; lane < total; ++lane) {
+      const int c = lane / n_seg, s = lane % n_seg;
+      const double* p = par + P_COUNT * c;
+      eng.apply_base_params(p);
+      State<double> st;
+      load37(st, seg_init + 37 * s);
+      double acc = 0.0;
+      for (int t = 0; t < H; ++t) {
+        double a[12];
+        for (int j = 0; j < 12; ++j) a[j] = seg_actions[(size_t(s) * H + t) * 12 + j];
+        eng.control_step(st, a, p);
+        const float* tg = seg_targets + (size_t(s) * H + t) * 37;
+        if (cost_mode == COST_TRACKING || t == H - 1) acc += step_cost<double>(cost_mode, w, st, tg);
+      }
+      if (!finite_state(st) || !std::isfinite(acc)) { acc = INFINITY; ++bad; }
+      cost_out[size_t(c) * n_seg + s] = acc;
+    }
+  }
+  return bad;
+}
+
 // Cost of one parameter vector on n_seg segments and its gradient with respect to idx[]:
 // cost_out [n_seg], grad_out [n_seg][P].
 int spio_cost_grad(void* hp, const double* par, const int* idx, int P, const float* seg_init, const float* seg_actions,

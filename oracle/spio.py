@@ -83,6 +83,11 @@ def lib(path: Optional[str] = None):
         ipp = C.POINTER(C.c_int)
         L.spio_traj_sens.argtypes = [vp, dp, ipp, C.c_int, fp, fp, C.c_int, C.c_int, dp, dp]
         L.spio_cost_grad.argtypes = [vp, dp, ipp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp, dp]
+        if hasattr(L, "spio_traj_sens_batch"):
+            L.spio_traj_sens_batch.argtypes = [vp, dp, ipp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp,
+                                               C.c_int]
+            L.spio_rollout_cost_par.argtypes = [vp, dp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp,
+                                                C.c_int]
         L.spio_rollout_cost_f32.argtypes = [vp, fp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, fp,
                                             C.c_int]
         L.spio_max_threads.restype = C.c_int
@@ -301,6 +306,32 @@ class Oracle:
                                   _fp(actions), T, n_rows, _dp(y), _dp(J))
         return y, J, rc
 
+    def traj_sens_batch(self, par, idx, init37, actions, n_rows=25, n_threads=0):
+        """traj_sens of one parameter vector over many trajectories, in parallel: y [n, T, n_rows],
+        J [n, T, n_rows, P], number of diverged trajectories."""
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        init37 = np.ascontiguousarray(init37, dtype=np.float32)
+        actions = np.ascontiguousarray(actions, dtype=np.float32)
+        n, T, P = actions.shape[0], actions.shape[1], idx.shape[0]
+        y, J = np.zeros((n, T, n_rows)), np.zeros((n, T, n_rows, P))
+        rc = self._L.spio_traj_sens_batch(self.h, _dp(par), idx.ctypes.data_as(C.POINTER(C.c_int)), P, _fp(init37),
+                                          _fp(actions), n, T, n_rows, _dp(y), _dp(J), int(n_threads))
+        return y, J, rc
+
+    def rollout_cost_par(self, par, seg_init, seg_actions, seg_targets, cost_mode, weights, n_threads=0):
+        """rollout_cost with the (candidate, segment) pairs over threads: [n_cand, n_seg]."""
+        par = np.ascontiguousarray(par, dtype=np.float64).reshape(-1, P_COUNT)
+        seg_init = np.ascontiguousarray(seg_init, dtype=np.float32)
+        seg_actions = np.ascontiguousarray(seg_actions, dtype=np.float32)
+        seg_targets = np.ascontiguousarray(seg_targets, dtype=np.float32)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        n_seg, H = seg_actions.shape[0], seg_actions.shape[1]
+        out = np.zeros((par.shape[0], n_seg))
+        self._L.spio_rollout_cost_par(self.h, _dp(par), par.shape[0], _fp(seg_init), _fp(seg_actions),
+                                      _fp(seg_targets), n_seg, H, cost_mode, _dp(w), _dp(out), int(n_threads))
+        return out
+
     def cost_grad(self, par, idx, seg_init, seg_actions, seg_targets, cost_mode, weights):
         """FP64 cost of one parameter vector per segment [n_seg] and its gradient [n_seg, P]."""
         par = np.ascontiguousarray(par, dtype=np.float64)
@@ -360,15 +391,13 @@ def gauss_newton_fp64(o: "Oracle", theta0, rows, segs, weights, seg_w, lo=None, 
     w = np.asarray(weights, np.float64)
     grp = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
 
+    seg_w = np.asarray(seg_w, np.float64)
+
     def resid(theta, with_jac):
-        R, Js = [], []
-        for s in range(segs.n_seg):
-            y, J, rc = o.traj_sens(theta, rows, segs.init[s], segs.actions[s], n_rows=37)
-            sw = np.sqrt(w[grp] * seg_w[s])
-            R.append((sw * (y - segs.targets[s].astype(np.float64))).ravel())
-            if with_jac:
-                Js.append((sw[None, :, None] * J).reshape(-1, len(rows)))
-        return np.concatenate(R), (np.concatenate(Js) if with_jac else None)
+        y, J, _ = o.traj_sens_batch(theta, rows, segs.init, segs.actions, n_rows=37)   # [n, T, 37(, P)]
+        sw = np.sqrt(w[grp][None, :] * seg_w[:, None])                                  # [n, 37]
+        R = (sw[:, None, :] * (y - segs.targets.astype(np.float64))).ravel()
+        return R, ((sw[:, None, :, None] * J).reshape(-1, len(rows)) if with_jac else None)
 
     theta = np.asarray(theta0, np.float64).copy()
     r, _ = resid(theta, False)

@@ -14,6 +14,8 @@
 using namespace spi;
 using namespace spi_adj_launch;
 using spi_launch::finite_bits;
+using spi_launch::lane_params_valid;
+using spi_launch::apply_segment_gains;
 using spi_launch::xcd_remap;
 
 namespace {
@@ -64,6 +66,7 @@ __global__ __launch_bounds__(kAdjThreads) void cost_adjoint_leg_kernel(
   __syncthreads();
   const int c = cb * (kAdjThreads / 4) + (threadIdx.x >> 2), l = threadIdx.x & 3;
   if (c >= n_cand) return;  // the four lanes of a rollout leave together
+  apply_segment_gains(a, cost, s);
   const int n_seg = gridDim.x / n_cblk;
   const size_t rollout = static_cast<size_t>(s) * n_cand + c;
   LegParams lp;
@@ -90,16 +93,20 @@ __global__ __launch_bounds__(kAdjThreads) void cost_adjoint_leg_kernel(
   LegParBar pb;
   zero_par(pb);
   float cst = 0.0f;
-  bool ok = rollout_grad_leg(a, cost, lp, K, G, raw, lds, H, every, l, store, ss, sa, cst, pb);
+  // the same verdict in the four lanes: an invalid candidate's quad skips the rollout (and its sums) together
+  const bool valid = lane_params_valid(par, ld, c, a.motor_model);
+  bool ok = valid && rollout_grad_leg(a, cost, lp, K, G, raw, lds, H, every, l, store, ss, sa, cst, pb);
+  if (valid) {
 #pragma unroll
-  for (int g = 0; g < 3; ++g) pb.motor[g] = sum4(pb.motor[g]);
+    for (int g = 0; g < 3; ++g) pb.motor[g] = sum4(pb.motor[g]);
+  }
   if (ok) {
     bool fin = finite_bits(cst) && finite_bits(pb.kp) && finite_bits(pb.kd) && finite_bits(pb.fric);
 #pragma unroll
     for (int k = 0; k < 10; ++k) fin = fin && finite_bits(pb.base[k]);
 #pragma unroll
     for (int g = 0; g < 3; ++g) fin = fin && finite_bits(pb.motor[g]);
-    ok = sum4(fin ? 0.0f : 1.0f) == 0.0f;   // the same verdict in the four lanes
+    ok = sum4(fin ? 0.0f : 1.0f) == 0.0f;   // the same verdict in the four lanes (ok implies valid)
   }
   float* gr = grad + rollout * SPI_P_COUNT;
   gr[10 + l] = ok ? pb.kp : 0.0f;

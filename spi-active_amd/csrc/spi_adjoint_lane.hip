@@ -15,6 +15,8 @@
 using namespace spi;
 using namespace spi_adj_launch;
 using spi_launch::finite_bits;
+using spi_launch::lane_params_valid;
+using spi_launch::apply_segment_gains;
 using spi_launch::xcd_remap;
 
 namespace {
@@ -54,6 +56,7 @@ __global__ __launch_bounds__(kAdjThreads) void cost_adjoint_kernel(
   __syncthreads();
   const int c = cb * kAdjThreads + threadIdx.x;
   if (c >= n_cand) return;
+  apply_segment_gains(a, cost, s);
   const int n_seg = gridDim.x / n_cblk;
   const size_t lane = static_cast<size_t>(s) * n_cand + c;
   LaneParams lp;
@@ -72,7 +75,8 @@ __global__ __launch_bounds__(kAdjThreads) void cost_adjoint_kernel(
   State ps[kMaxDecimation], ss[kMaxSubsteps];
   ParBar pb;
   float cst = 0.0f;
-  bool ok = rollout_grad(a, cost, lp, raw, lds, H, every, store, ps, ss, cst, pb);
+  bool ok = lane_params_valid(par, ld, c, a.motor_model) &&
+            rollout_grad(a, cost, lp, raw, lds, H, every, store, ps, ss, cst, pb);
   if (ok) {
     ok = finite_bits(cst);
 #pragma unroll

@@ -225,9 +225,53 @@ __global__ __launch_bounds__(kCemThreads) void cem_update_batch_kernel(SpiBatchP
   if (threadIdx.x == 0) P.iteration = P.iteration + 1u;
 }
 
+// Reset draw of one env: uniforms u_k, k = 0..17, word k % 4 of Philox block k / 4 of the
+// counter (env, block, 'RSET') under key seed.  q_j = q_default_j * (q_lo + (q_hi - q_lo) u_j),
+// qd = 0; base at root0's position (+ the env's origin) and orientation; linear then angular
+// velocity -v + 2 v u_{12+i}.  An env's draw depends only on (seed, env).
+__global__ void reset_states_kernel(SpiResetSpec spec, int n_env, const float* __restrict__ origins,
+                                    unsigned long long seed, float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_env) return;
+  float u[20];
+#pragma unroll
+  for (int b = 0; b < 5; ++b) {
+    uint32_t c[4] = {static_cast<uint32_t>(e), 0u, static_cast<uint32_t>(b), 0x52534554u};
+    philox4x32_10(c, static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32));
+#pragma unroll
+    for (int w = 0; w < 4; ++w) u[4 * b + w] = u01_open(c[w]);
+  }
+  float* o = out + static_cast<size_t>(e) * SPI_STATE_DIM;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) o[i] = spec.root0[i];
+  if (origins) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) o[i] += origins[3 * static_cast<size_t>(e) + i];
+  }
+  const float v = spec.vel_range;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) o[7 + i] = -v + 2.0f * v * u[12 + i];
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    o[13 + j] = spec.q_default[j] * (spec.q_lo + (spec.q_hi - spec.q_lo) * u[j]);
+    o[25 + j] = 0.0f;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int spi_reset_states(const SpiResetSpec* spec, int n_env, const float* origins_dev, unsigned long long seed,
+                     float* out_dev, hipStream_t stream) {
+  if (!spec || n_env < 0 || !out_dev || !(spec->q_lo <= spec->q_hi) || !(spec->vel_range >= 0.0f))
+    return cem_fail(SPI_ERR_INVALID_ARG, "spi_reset_states: bad arguments");
+  if (n_env == 0) return SPI_OK;
+  reset_states_kernel<<<(n_env + 255) / 256, 256, 0, stream>>>(*spec, n_env, origins_dev, seed, out_dev);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return cem_fail(SPI_ERR_HIP, hipGetErrorString(e));
+  return SPI_OK;
+}
 
 int spi_cem_sample(const float* mean_dev, const float* std_dev, const float* lo_dev, const float* hi_dev, int n_param,
                    int n, unsigned long long seed, unsigned int iteration, float* theta_dev, int ld,

@@ -30,6 +30,7 @@ int hip_check(hipError_t e, const char* what) {
 }
 int launch_check(const char* what) { return hip_check(hipGetLastError(), what); }
 
+using spi_launch::apply_segment_gains;
 using spi_launch::xcd_remap;
 
 __device__ __forceinline__ LaneParams load_params(const float* __restrict__ par, int ld, int i) {
@@ -48,6 +49,7 @@ __device__ __forceinline__ LaneParams load_params(const float* __restrict__ par,
 }
 
 using spi_launch::finite_bits;
+using spi_launch::lane_params_valid;
 
 constexpr int kRolloutThreads = 256;
 
@@ -94,8 +96,11 @@ __global__ __launch_bounds__(kRolloutThreads) void rollout_cost_kernel(
   __syncthreads();
   const int c = cb * kRolloutThreads + threadIdx.x;
   if (c >= n_cand) return;
+  apply_segment_gains(a, cost, s);
+  unsigned int* o = reinterpret_cast<unsigned int*>(out) + static_cast<size_t>(s) * n_cand + c;
+  if (!lane_params_valid(par, ld, c, a.motor_model)) { *o = 0x7f800000u; return; }
   const LaneParams lp = load_params(par, ld, c);
-  reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] = rollout_lane<1>(a, cost, lp, lds, H);
+  *o = rollout_lane<1>(a, cost, lp, lds, H);
 }
 
 // rollout_cost_kernel with four lanes per candidate, one per leg (spi_dyn_leg.h): one
@@ -116,6 +121,11 @@ __global__ __launch_bounds__(kRolloutThreads) void rollout_cost_leg_kernel(
   __syncthreads();
   const int c = cb * (kRolloutThreads / 4) + (threadIdx.x >> 2), l = threadIdx.x & 3;
   if (c >= n_cand) return;  // the four lanes of a candidate leave together
+  apply_segment_gains(a, cost, s);
+  if (!lane_params_valid(par, ld, c, a.motor_model)) {   // the same verdict in the four lanes
+    if (l == 0) reinterpret_cast<unsigned int*>(out)[static_cast<size_t>(s) * n_cand + c] = 0x7f800000u;
+    return;
+  }
   const LaneParams full = load_params(par, ld, c);
   LegParams lp;
   lp.base = full.base;
@@ -187,9 +197,15 @@ __global__ __launch_bounds__(BLOCK) void rollout_cost_batch_kernel(SpiSimArgs a,
     __syncthreads();
     const int c = cb * BLOCK + threadIdx.x;
     if (c < P.n_samples) {
-      const LaneParams lp = load_params(P.theta, P.n_samples, c);
-      reinterpret_cast<unsigned int*>(P.cost_seg)[static_cast<size_t>(s) * P.n_samples + c] =
-          rollout_lane<UNROLL>(a, P.cost, lp, lds, P.horizon);
+      SpiSimArgs at = a;
+      apply_segment_gains(at, P.cost, s);
+      unsigned int* o = reinterpret_cast<unsigned int*>(P.cost_seg) + static_cast<size_t>(s) * P.n_samples + c;
+      if (lane_params_valid(P.theta, P.n_samples, c, at.motor_model)) {
+        const LaneParams lp = load_params(P.theta, P.n_samples, c);
+        *o = rollout_lane<UNROLL>(at, P.cost, lp, lds, P.horizon);
+      } else {
+        *o = 0x7f800000u;
+      }
     }
     __syncthreads();                          // LDS and `ticket` are reused by the next task
   }
@@ -459,8 +475,9 @@ __global__ __launch_bounds__(256) void fim_rollout_kernel(SpiSimArgs a, const fl
     if (member == 0 || member > P) e = 0.0f;
     for (int off = 1; off < G2; off <<= 1) e += __shfl_xor(e, off, 64);
     acc += e;
-    const int step = t + 1;
-    if (ksync > 0 && step % ksync == (ksync == 1 ? 0 : 1)) {
+    // the reference's step_idx of this command step: reset_all's zero-action step is step 1
+    const int step_idx = t + 2;
+    if (ksync > 0 && (ksync == 1 || step_idx % ksync == 1)) {
       if (ksync_mode == 1) {
         // full state copy from the main env
         st.p.x = __shfl(st.p.x, main_lane, 64); st.p.y = __shfl(st.p.y, main_lane, 64);

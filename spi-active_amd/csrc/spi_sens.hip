@@ -24,6 +24,7 @@ SPI_DUAL_MATH(spi_tan, 1)
 
 using namespace spi_tan;
 using spi_launch::finite_bits;
+using spi_launch::apply_segment_gains;
 using spi_launch::xcd_remap;
 
 #include "spi_sens_common.h"
@@ -144,6 +145,7 @@ __global__ __launch_bounds__(kSensThreads) void residual_sens_leg_kernel(SpiSimA
   const long long g = u / L;
   const int k = static_cast<int>(u % L);
   const int c = static_cast<int>(g / n_seg), s = static_cast<int>(g % n_seg);
+  apply_segment_gains(a, cost, s);
   const float* rec0 = seg + static_cast<size_t>(s) * stride;
   const LaneParams full = load_params_tan(par, ld, c, k < P ? tan.param[k] : -1);
   LegParams lp;

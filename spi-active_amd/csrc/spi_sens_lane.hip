@@ -24,6 +24,7 @@ SPI_DUAL_MATH(spi_tan, 1)
 
 using namespace spi_tan;
 using spi_launch::finite_bits;
+using spi_launch::apply_segment_gains;
 using spi_launch::xcd_remap;
 
 #include "spi_sens_common.h"
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(kSensThreads) void cost_grad_kernel(SpiSimArgs a, S
   const int gl = threadIdx.x / P, k = threadIdx.x % P;
   const int c = cb * gpb + gl;
   if (gl >= gpb || c >= n_cand) return;
+  apply_segment_gains(a, cost, s);
   const LaneParams lp = load_params_tan(par, ld, c, tan.param[k]);
   State st;
   init_state(st, lds);
@@ -153,6 +155,7 @@ __global__ __launch_bounds__(kSensThreads) void residual_sens_kernel(SpiSimArgs 
   const long long g = static_cast<long long>(blockIdx.x) * gpb + gl;
   if (gl >= gpb || g >= static_cast<long long>(n_cand) * n_seg) return;
   const int c = static_cast<int>(g / n_seg), s = static_cast<int>(g % n_seg);
+  apply_segment_gains(a, cost, s);
   const float* rec0 = seg + static_cast<size_t>(s) * stride;
   const LaneParams lp = load_params_tan(par, ld, c, k < P ? tan.param[k] : -1);
   State st;

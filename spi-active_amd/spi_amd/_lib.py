@@ -31,7 +31,12 @@ class SpiSimArgs(C.Structure):
 
 
 class SpiCostArgs(C.Structure):
-    _fields_ = [("mode", C.c_int), ("w", C.c_float * 6)]
+    _fields_ = [("mode", C.c_int), ("w", C.c_float * 6), ("seg_gains", C.c_void_p)]
+
+
+class SpiResetSpec(C.Structure):
+    _fields_ = [("q_default", C.c_float * 12), ("root0", C.c_float * 7), ("q_lo", C.c_float), ("q_hi", C.c_float),
+                ("vel_range", C.c_float)]
 
 
 class SpiTangentSpec(C.Structure):
@@ -74,10 +79,18 @@ def sim_args(sim: SimParams, state_limit: float = 1.0e4) -> SpiSimArgs:
     return a
 
 
-def cost_args(mode: int, weights) -> SpiCostArgs:
+def cost_args(mode: int, weights, seg_gains: Optional[torch.Tensor] = None) -> SpiCostArgs:
+    """Cost settings; ``seg_gains`` is an optional device [n_seg, 2] float32 tensor of recorded
+    (kp, kd) per segment (the caller keeps it alive while launches that use it are queued)."""
     c = SpiCostArgs()
     c.mode = int(mode)
     c.w[:] = [float(x) for x in weights]
+    c.seg_gains = None
+    if seg_gains is not None:
+        if seg_gains.dtype != torch.float32 or seg_gains.dim() != 2 or seg_gains.shape[1] != 2 \
+                or not seg_gains.is_contiguous():
+            raise ValueError("seg_gains must be a contiguous float32 [n_seg, 2] tensor")
+        c.seg_gains = seg_gains.data_ptr()
     return c
 
 
@@ -139,6 +152,7 @@ def lib() -> C.CDLL:
         "spi_policy_reward_terms": [ip, ip, vp, vp, ip, ip, ip, fp, vp, vp, vp, vp, ip, ip, ip, fp, fp, fp, fp, fp, fp,
                                     vp, vp, vp, vp, vp],
         "spi_policy_obs_kmap": [C.POINTER(ip)],
+        "spi_reset_states": [C.POINTER(SpiResetSpec), ip, vp, C.c_ulonglong, vp, vp],
     }
     L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
     L.spi_adjoint_legs_work_floats.argtypes = [ip, ip, ip, ip, ip]
@@ -154,7 +168,7 @@ def lib() -> C.CDLL:
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ip
-    if L.spi_abi_version() != 1:
+    if L.spi_abi_version() != 2:
         raise RuntimeError("libspi_hip.so ABI version mismatch")
     _lib = L
     return L

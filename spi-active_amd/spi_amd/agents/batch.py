@@ -46,6 +46,7 @@ class ProblemSpec:
     alpha: float = 0.0
     seed: int = 0
     seg_weights: Optional[torch.Tensor] = None
+    seg_gains: Optional[torch.Tensor] = None   # recorded (kp, kd) per segment [n_seg, 2], or configured gains
 
 
 def _validate(k: int, s: ProblemSpec) -> None:
@@ -66,6 +67,10 @@ def _validate(k: int, s: ProblemSpec) -> None:
     if s.seg_weights is not None and (s.seg_weights.numel() != s.segments.shape[0] or
                                       s.seg_weights.dtype != torch.float32 or s.seg_weights.device.type != "cuda"):
         raise ValueError(f"problem {k}: seg_weights must be float32 [n_seg] on the device")
+    if s.seg_gains is not None and (tuple(s.seg_gains.shape) != (s.segments.shape[0], 2) or
+                                    s.seg_gains.dtype != torch.float32 or not s.seg_gains.is_contiguous() or
+                                    s.seg_gains.device.type != "cuda"):
+        raise ValueError(f"problem {k}: seg_gains must be a contiguous float32 [n_seg, 2] device tensor")
 
 
 _PRIORS: Dict[int, tuple] = {}
@@ -128,6 +133,8 @@ def pack_problem_table(specs, bufs, start_iter, cargs, block: int, dev):
     arr["horizon"] = [s.horizon for s in specs]
     arr["cost"]["mode"] = cargs.mode
     arr["cost"]["w"] = np.array(list(cargs.w), np.float32)
+    arr["cost"]["seg_gains"] = np.array([0 if s.seg_gains is None else s.seg_gains.data_ptr() for s in specs],
+                                        np.uint64)
     arr["n_samples"] = n_samp
     arr["n_elite"] = [max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9))) for s in specs]
     arr["alpha"] = [s.alpha for s in specs]

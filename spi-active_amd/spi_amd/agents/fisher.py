@@ -146,7 +146,7 @@ class FisherObjective:
 
 
 def adjoint_cost_grad(engine: RolloutEngine, theta: torch.Tensor, segments: torch.Tensor, horizon: int,
-                      checkpoint_every: int = 10, legs: bool = True):
+                      checkpoint_every: int = 10, legs: bool = True, seg_gains: Optional[torch.Tensor] = None):
     """Cost [n_seg, n] and gradient [n_seg, n, 25] (all parameter rows) by reverse mode with
     checkpoints every ``checkpoint_every`` control steps: four lanes per rollout
     (``spi_rollout_cost_adjoint_legs``, the default: no register spills, 1.6x faster at
@@ -168,7 +168,8 @@ def adjoint_cost_grad(engine: RolloutEngine, theta: torch.Tensor, segments: torc
     work = torch.empty(max(wf, 1), dtype=torch.float32, device=dev)
     cost = torch.empty((n_seg, n), dtype=torch.float32, device=dev)
     grad = torch.empty((n_seg, n, P_COUNT), dtype=torch.float32, device=dev)
-    rc = getattr(L, name)(C_ref(engine.args), C_ref(engine.cargs), _lib.ptr(theta), n, n, _lib.ptr(segments), n_seg,
+    cargs = engine.cost_args_for(n_seg, seg_gains)
+    rc = getattr(L, name)(C_ref(engine.args), C_ref(cargs), _lib.ptr(theta), n, n, _lib.ptr(segments), n_seg,
                           horizon, checkpoint_every, _lib.ptr(work), wf, _lib.ptr(cost), _lib.ptr(grad),
                           _lib.stream())
     _lib.check(rc, name)
@@ -185,11 +186,12 @@ class RolloutCost(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, theta: torch.Tensor, engine: RolloutEngine, segments: torch.Tensor, horizon: int,
-                seg_weights: torch.Tensor, rows: Sequence[int], method: str = "adjoint", checkpoint_every: int = 10):
+                seg_weights: torch.Tensor, rows: Sequence[int], method: str = "adjoint", checkpoint_every: int = 10,
+                seg_gains: Optional[torch.Tensor] = None):
         n = theta.shape[1]
         theta = theta.detach().contiguous()
         if method == "adjoint":
-            cost, grad = adjoint_cost_grad(engine, theta, segments, horizon, checkpoint_every)
+            cost, grad = adjoint_cost_grad(engine, theta, segments, horizon, checkpoint_every, seg_gains=seg_gains)
             w = seg_weights.to(torch.float32)
             total = (w[:, None] * cost).sum(0)
             full = torch.zeros((P_COUNT, n), dtype=torch.float32, device=theta.device)
@@ -205,7 +207,8 @@ class RolloutCost(torch.autograd.Function):
             raise ValueError("segment stride does not match the horizon")
         cost = torch.empty((n_seg, n), dtype=torch.float32, device=theta.device)
         grad = torch.empty((n_seg, n, spec.n_tan), dtype=torch.float32, device=theta.device)
-        rc = _lib.lib().spi_rollout_cost_grad(C_ref(engine.args), C_ref(engine.cargs), C_ref(spec), _lib.ptr(theta),
+        cargs = engine.cost_args_for(n_seg, seg_gains)
+        rc = _lib.lib().spi_rollout_cost_grad(C_ref(engine.args), C_ref(cargs), C_ref(spec), _lib.ptr(theta),
                                               n, n, _lib.ptr(segments), n_seg, horizon, _lib.ptr(cost),
                                               _lib.ptr(grad), _lib.stream())
         _lib.check(rc, "spi_rollout_cost_grad")
@@ -220,4 +223,4 @@ class RolloutCost(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g_out):
         (full,) = ctx.saved_tensors
-        return full * g_out[None, :], None, None, None, None, None, None, None
+        return full * g_out[None, :], None, None, None, None, None, None, None, None

@@ -163,6 +163,32 @@ def reset_states(n_env: int, q_default: np.ndarray, seed: int = 0, base_height: 
     return s
 
 
+def reset_states_device(n_env: int, q_default: np.ndarray, seed: int = 0, base_height: float = 0.34,
+                        vel_range: float = 0.5, q_range=(0.5, 1.5), origins: Optional[torch.Tensor] = None,
+                        out: Optional[torch.Tensor] = None, device="cuda") -> torch.Tensor:
+    """The reset draw of ``reset_states`` on the device (``spi_reset_states``): joint positions
+    q_default * U[0.5, 1.5], zero joint velocities (active_sysid_openloop.py:274-292), base at
+    (0, 0, base_height) (+ origins [n_env, 3]) with identity orientation and linear and angular
+    velocity U[-0.5, 0.5] (legged_robot_base.py:761-784).  Philox4x32-10 under ``seed``: env e's
+    draw depends only on (seed, e), so a draw can be repeated for any subset.  [n_env, 37]."""
+    spec = _lib.SpiResetSpec()
+    spec.q_default[:] = [float(x) for x in np.asarray(q_default, np.float64).reshape(12)]
+    spec.root0[:] = [0.0, 0.0, float(base_height), 0.0, 0.0, 0.0, 1.0]
+    spec.q_lo, spec.q_hi = float(q_range[0]), float(q_range[1])
+    spec.vel_range = float(vel_range)
+    if out is None:
+        out = torch.empty((n_env, 37), dtype=torch.float32, device=device)
+    if out.shape != (n_env, 37) or not out.is_contiguous() or out.dtype != torch.float32:
+        raise ValueError(f"out must be a contiguous float32 [{n_env}, 37] tensor")
+    if origins is not None and (origins.shape != (n_env, 3) or not origins.is_contiguous()
+                                or origins.dtype != torch.float32):
+        raise ValueError(f"origins must be a contiguous float32 [{n_env}, 3] tensor")
+    rc = _lib.lib().spi_reset_states(C.byref(spec), n_env, _lib.ptr(origins), int(seed), _lib.ptr(out),
+                                     _lib.stream())
+    _lib.check(rc, "spi_reset_states")
+    return out
+
+
 class PolicyFIMRollout:
     """Scores command sequences by the FD Fisher information of a policy-driven rollout."""
 
@@ -422,7 +448,11 @@ class PolicyActiveSysId:
     best trial at the end."""
 
     def __init__(self, rollout: PolicyFIMRollout, generator, batch_size: int = 1024, elite_frac: float = 0.1,
-                 seed: int = 0, init: Optional[torch.Tensor] = None, graph: bool = True):
+                 seed: int = 0, init: Optional[torch.Tensor] = None, graph: bool = True, num_rollouts: int = 1):
+        """init: fixed initial states [batch_size * (P + 1), 37] for every evaluation, or None
+        (the reference's behaviour): every rollout starts from a fresh reset draw on the device,
+        as evaluate_policy resets the envs before each of its num_rollouts_per_command rollouts
+        (active_sysid.py:540-543) and averages their rewards (:594-597)."""
         from .cem import CEMConfig, DeviceCEM
         self.rollout, self.gen, self.graph = rollout, generator, graph
         bounds = np.asarray(generator.param_bounds(), np.float64)
@@ -430,28 +460,61 @@ class PolicyActiveSysId:
         self.names = generator.param_names()
         self.cem = DeviceCEM(0.5 * (lo + hi), 0.5 * (hi - lo), lo, hi, np.maximum(1e-3 * (hi - lo), 0.0),
                              CEMConfig(n_samples=batch_size, elite_frac=elite_frac, seed=seed))
-        if init is None:
-            n_env = batch_size * rollout.G
-            init = torch.from_numpy(reset_states(n_env, rollout.engine.sim.q0, seed=seed)).to(rollout.engine.device)
+        if num_rollouts < 1:
+            raise ValueError("num_rollouts must be >= 1")
+        self.num_rollouts = int(num_rollouts)
+        self.seed = int(seed)
+        self.n_env = batch_size * rollout.G
         self.init = init
+        self._reset_buf = None
+        self.draws = 0                  # reset draws so far; draw k uses Philox key reset_key(k)
         self.history = []
+
+    def reset_key(self, draw: int) -> int:
+        """Philox key of reset draw ``draw``: the optimizer's seed in the low word, the draw in the high."""
+        return (self.seed & 0xFFFFFFFF) | (int(draw) << 32)
+
+    def _initial_states(self) -> torch.Tensor:
+        if self.init is not None:
+            return self.init
+        if self._reset_buf is None:
+            self._reset_buf = torch.empty((self.n_env, 37), dtype=torch.float32, device=self.rollout.engine.device)
+        reset_states_device(self.n_env, self.rollout.engine.sim.q0, seed=self.reset_key(self.draws),
+                            out=self._reset_buf)
+        self.draws += 1
+        return self._reset_buf
 
     def step(self) -> torch.Tensor:
         theta = self.cem.ask()                                 # [n_params, B]
         cmds = self.gen.generate(theta.t())                    # [B, total_steps, 14]
-        fim = self.rollout.evaluate(cmds, self.init, graph=self.graph)
+        fim = None
+        for _ in range(self.num_rollouts):
+            f = self.rollout.evaluate(cmds, self._initial_states(), graph=self.graph)
+            fim = f.clone() if fim is None else fim + f
+        fim = fim / float(self.num_rollouts)
         cost = torch.nan_to_num(-fim, nan=float("inf"))
         self.cem.tell(cost)
         self.history.append(fim)
         return fim
 
-    def optimize(self, iterations: int) -> Dict:
+    def optimize(self, iterations: int, log_dir: Optional[str] = None) -> Dict:
+        """``iterations`` optimizer iterations; the best trial's parameters, value (the negated
+        FIM, what the reference's study minimises) and command sequence.  With ``log_dir`` the
+        sequence is also saved as ``log_dir/best_commands.npz`` (key ``best_commands``), as
+        active_sysid.py:229-231 does."""
+        import os
         for _ in range(iterations):
             self.step()
         best = self.cem.best.cpu().numpy().astype(np.float64)
         params = dict(zip(self.names, best.tolist()))
-        return {"best_params": params, "best_value": float(self.cem.best_cost.item()),
-                "best_commands": self.gen.from_params(params)}
+        out = {"best_params": params, "best_value": float(self.cem.best_cost.item()),
+               "best_commands": self.gen.from_params(params)}
+        if log_dir is not None:
+            os.makedirs(log_dir, exist_ok=True)
+            path = os.path.join(log_dir, "best_commands.npz")
+            np.savez(path, best_commands=out["best_commands"])
+            out["best_commands_path"] = path
+        return out
 
 
 # params_dict keys of the active-sysID env (env/active_sysid_openloop.yaml default_param) -> parameter
@@ -515,4 +578,5 @@ def active_sysid_from_config(cfg: Dict, policy: PolicyMLP, engine: Optional[Roll
     sampler = algo.get("optimize", {}).get("sampler", {})
     if "batch_size" in sampler:
         batch = int(sampler["batch_size"])
-    return PolicyActiveSysId(roll, gen, batch_size=batch, seed=seed)
+    return PolicyActiveSysId(roll, gen, batch_size=batch, seed=seed,
+                             num_rollouts=int(cmd.get("num_rollouts_per_command", 1)))

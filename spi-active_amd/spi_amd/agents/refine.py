@@ -35,7 +35,8 @@ class RefineResult:
 class GaussNewtonRefiner:
     def __init__(self, engine: RolloutEngine, segments: torch.Tensor, horizon: int,
                  seg_weights: Optional[torch.Tensor] = None, names: Sequence[str] = IDENTIFIED,
-                 lo: Optional[np.ndarray] = None, hi: Optional[np.ndarray] = None, legs: bool = True):
+                 lo: Optional[np.ndarray] = None, hi: Optional[np.ndarray] = None, legs: bool = True,
+                 seg_gains: Optional[torch.Tensor] = None):
         if engine.cost_mode != 0:
             raise ValueError("Gauss-Newton refinement needs the tracking (sum of squares) cost")
         if segments.shape[1] != segment_stride(horizon):
@@ -44,6 +45,8 @@ class GaussNewtonRefiner:
         # residual Jacobian with four lanes per tangent (one per leg) or one lane per tangent
         self.jacobian_entry = "spi_residual_jacobian_legs" if legs else "spi_residual_jacobian"
         self.n_seg = segments.shape[0]
+        self.seg_gains = seg_gains
+        self.cargs = engine.cost_args_for(self.n_seg, seg_gains)
         dev = engine.device
         self.seg_w = seg_weights if seg_weights is not None else \
             torch.full((self.n_seg,), 1.0 / self.n_seg, dtype=torch.float32, device=dev)
@@ -67,14 +70,14 @@ class GaussNewtonRefiner:
     def objective(self, theta: np.ndarray) -> float:
         """Tracking cost sum_s w_s cost_s of one parameter vector (fused primal kernel)."""
         t = self._theta_dev(theta)
-        c = self.engine.segment_costs(t, 1, self.segments, self.H)
+        c = self.engine.segment_costs(t, 1, self.segments, self.H, seg_gains=self.seg_gains)
         return float((self.seg_w[:, None] * c).sum().item())
 
     def step(self, theta: np.ndarray, lam: float):
         """One damped Gauss-Newton step: (delta [P], cost at theta, predicted cost after it)."""
         t = self._theta_dev(theta)
         L = _lib.lib()
-        rc = getattr(L, self.jacobian_entry)(C_ref(self.engine.args), C_ref(self.engine.cargs), C_ref(self.spec),
+        rc = getattr(L, self.jacobian_entry)(C_ref(self.engine.args), C_ref(self.cargs), C_ref(self.spec),
                                              _lib.ptr(t), 1, 1, _lib.ptr(self.segments), _lib.ptr(self.seg_w),
                                              self.n_seg, self.H, _lib.ptr(self.Jr), _lib.ptr(self.status, torch.int32),
                                              _lib.stream())

@@ -58,8 +58,10 @@ class SysIdProblem:
     """Batched objective: SoA candidates [25, n] -> mean tracking cost over the segments [n]."""
 
     def __init__(self, engine: RolloutEngine, segments: torch.Tensor, horizon: int,
-                 seg_weights: Optional[torch.Tensor] = None, max_cand: int = 8192):
+                 seg_weights: Optional[torch.Tensor] = None, max_cand: int = 8192,
+                 seg_gains: Optional[torch.Tensor] = None):
         self.engine, self.segments, self.horizon = engine, segments, horizon
+        self.seg_gains = seg_gains     # recorded (kp, kd) per segment, or None: configured gains
         n_seg = segments.shape[0]
         dev = engine.device
         self.seg_weights = seg_weights if seg_weights is not None else \
@@ -70,7 +72,7 @@ class SysIdProblem:
     def __call__(self, theta: torch.Tensor, n: int) -> torch.Tensor:
         sc = self._seg_cost[:, :n] if n == self._seg_cost.shape[1] else \
             self._seg_cost.view(-1)[: self.segments.shape[0] * n].view(self.segments.shape[0], n)
-        self.engine.segment_costs(theta, n, self.segments, self.horizon, out=sc)
+        self.engine.segment_costs(theta, n, self.segments, self.horizon, out=sc, seg_gains=self.seg_gains)
         return self.engine.reduce(sc, self.seg_weights, out=self._cost[:n])
 
 
@@ -80,7 +82,8 @@ COST_COEFF = {"base_pos": 10.0, "base_quat": 5.0, "joint_pos": 1.0}
 
 
 def mass_landscape(engine: RolloutEngine, segments: torch.Tensor, horizon: int, base_params: np.ndarray,
-                   masses: np.ndarray, seg_weights: Optional[torch.Tensor] = None) -> Tuple[np.ndarray, int]:
+                   masses: np.ndarray, seg_weights: Optional[torch.Tensor] = None,
+                   seg_gains: Optional[torch.Tensor] = None) -> Tuple[np.ndarray, int]:
     """Cost of every base mass on a grid, all candidates in one fused launch.
 
     Counterpart of ``mass_sweep`` + argmin (scripts/mass_landscape.py:111-171), which
@@ -92,6 +95,6 @@ def mass_landscape(engine: RolloutEngine, segments: torch.Tensor, horizon: int, 
     rows = np.repeat(np.asarray(base_params, np.float64)[None], len(masses), axis=0)
     rows[:, P_MASS] = masses
     theta = torch.from_numpy(np.ascontiguousarray(rows.T.astype(np.float32))).to(engine.device)
-    prob = SysIdProblem(engine, segments, horizon, seg_weights, max_cand=len(masses))
+    prob = SysIdProblem(engine, segments, horizon, seg_weights, max_cand=len(masses), seg_gains=seg_gains)
     c = prob(theta, len(masses)).cpu().numpy()
     return c, int(np.argmin(c))

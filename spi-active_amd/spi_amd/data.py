@@ -51,6 +51,17 @@ class SegmentSet:
         out[:, STATE_DIM:] = rec
         return out
 
+    @staticmethod
+    def from_packed(packed: np.ndarray, horizon: int) -> "SegmentSet":
+        """Inverse of ``packed()`` (no motion ends, no recorded gains)."""
+        packed = np.asarray(packed, np.float32)
+        n = packed.shape[0]
+        if packed.ndim != 2 or packed.shape[1] != segment_stride(horizon):
+            raise ValueError("packed segments do not match the horizon")
+        rec = packed[:, STATE_DIM:].reshape(n, horizon, 12 + STATE_DIM)
+        return SegmentSet(packed[:, :STATE_DIM].copy(), rec[:, :, :12].copy(), rec[:, :, 12:].copy(),
+                          np.zeros(n, dtype=bool))
+
     def subset(self, idx) -> "SegmentSet":
         return SegmentSet(self.init[idx], self.actions[idx], self.targets[idx], self.motion_ends[idx],
                           None if self.kp is None else self.kp[idx], None if self.kd is None else self.kd[idx])
@@ -174,6 +185,31 @@ def reference_segment_weights(motion_ends: np.ndarray, batch_size: int) -> np.nd
         mask = ~(np.cumsum(m.astype(np.float64)) > 0)
         w[start:start + batch_size] = mask / total_valid
     return w.astype(np.float32)
+
+
+def segment_gains(kp: np.ndarray, kd: np.ndarray, batch_size: Optional[int] = None) -> np.ndarray:
+    """Recorded PD gains per segment, [n, 2] float32 (kp, kd), for ``SpiCostArgs.seg_gains``.
+
+    ``kp``/``kd`` are per-segment (``[n]``) or per-segment-and-joint (``[n, 12]``, all joints
+    equal, as load_npz_dataset fills them; column 0 is used).  Without ``batch_size`` every
+    segment keeps the gains of its own file.  With ``batch_size`` the reference's chunking is
+    reproduced: scripts/eval.py:245-250 evaluates the windows in chunks of ``num_envs`` and
+    apply_pd_gains_batch (:192-201) sets the env's gains from the chunk's FIRST window, so a
+    chunk that straddles two files runs all of its windows with the first file's gains.
+    """
+    kp = np.asarray(kp, np.float32)
+    kd = np.asarray(kd, np.float32)
+    kp = kp[:, 0] if kp.ndim == 2 else kp
+    kd = kd[:, 0] if kd.ndim == 2 else kd
+    if kp.shape != kd.shape or kp.ndim != 1:
+        raise ValueError("kp and kd must have one entry (or one row) per segment")
+    g = np.stack([kp, kd], axis=1)
+    if batch_size is not None:
+        if batch_size < 1:
+            raise ValueError("batch_size must be >= 1")
+        first = (np.arange(g.shape[0]) // batch_size) * batch_size
+        g = g[first]
+    return np.ascontiguousarray(g, np.float32)
 
 
 def save_npz_trajectory(path: str, states: np.ndarray, actions: np.ndarray, dt: float, kp: float, kd: float) -> None:

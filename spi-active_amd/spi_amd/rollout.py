@@ -45,12 +45,24 @@ class RolloutEngine:
     def upload_segments(self, segs: SegmentSet) -> torch.Tensor:
         return torch.from_numpy(segs.packed()).to(self.device)
 
+    def cost_args_for(self, n_seg: int, seg_gains: Optional[torch.Tensor] = None):
+        """The engine's cost settings, with the recorded PD gains of each segment when given
+        (device [n_seg, 2] float32 (kp, kd), e.g. from ``data.segment_gains``)."""
+        if seg_gains is None:
+            return self.cargs
+        if tuple(seg_gains.shape) != (n_seg, 2) or not seg_gains.is_cuda:
+            raise ValueError(f"seg_gains must be a [{n_seg}, 2] device tensor")
+        return _lib.cost_args(self.cost_mode, self.weights, seg_gains)
+
     def segment_costs(self, params: torch.Tensor, n_cand: int, segs: torch.Tensor, horizon: int,
-                      out: Optional[torch.Tensor] = None, physics: str = "auto") -> torch.Tensor:
+                      out: Optional[torch.Tensor] = None, physics: str = "auto",
+                      seg_gains: Optional[torch.Tensor] = None) -> torch.Tensor:
         """Cost of every (segment, candidate) pair: [n_seg, n_cand].  physics: "lanes" (one
         rollout per lane), "legs" (four lanes per rollout: lower latency when the batch leaves
-        SIMDs idle) or "auto" (legs up to LEG_PARALLEL_MAX_ROLLOUTS rollouts)."""
+        SIMDs idle) or "auto" (legs up to LEG_PARALLEL_MAX_ROLLOUTS rollouts).  seg_gains:
+        recorded (kp, kd) per segment in place of the configured PD gains."""
         n_seg = segs.shape[0]
+        cargs = self.cost_args_for(n_seg, seg_gains)
         if segs.shape[1] != segment_stride(horizon):
             raise ValueError("segment tensor stride does not match the horizon")
         if params.dim() != 2 or params.shape[0] != P_COUNT or params.shape[1] < n_cand:
@@ -61,7 +73,7 @@ class RolloutEngine:
             raise ValueError("physics must be 'auto', 'lanes' or 'legs'")
         legs = physics == "legs" or (physics == "auto" and n_cand * n_seg <= LEG_PARALLEL_MAX_ROLLOUTS)
         fn = self.L.spi_rollout_cost_legs if legs else self.L.spi_rollout_cost
-        rc = fn(C_ref(self.args), C_ref(self.cargs), _lib.ptr(params), params.shape[1], n_cand, _lib.ptr(segs), n_seg,
+        rc = fn(C_ref(self.args), C_ref(cargs), _lib.ptr(params), params.shape[1], n_cand, _lib.ptr(segs), n_seg,
                 horizon, _lib.ptr(out), _lib.stream())
         _lib.check(rc, "spi_rollout_cost")
         return out
@@ -76,9 +88,9 @@ class RolloutEngine:
         return out
 
     def costs(self, params: torch.Tensor, n_cand: int, segs: torch.Tensor, horizon: int,
-              seg_weights: Optional[torch.Tensor] = None) -> torch.Tensor:
+              seg_weights: Optional[torch.Tensor] = None, seg_gains: Optional[torch.Tensor] = None) -> torch.Tensor:
         """Per-candidate cost (mean over segments unless weights are given)."""
-        sc = self.segment_costs(params, n_cand, segs, horizon)
+        sc = self.segment_costs(params, n_cand, segs, horizon, seg_gains=seg_gains)
         if seg_weights is None:
             seg_weights = torch.full((segs.shape[0],), 1.0 / segs.shape[0], dtype=torch.float32, device=self.device)
         return self.reduce(sc, seg_weights)

@@ -1,0 +1,101 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/c1_fp64_minimiser.npz: the 22-parameter CEM workload's data and its FP64
+least-squares minimiser, both from the FP64 oracle (oracle/spi_oracle.hpp), for the
+identification-parity test (tests/test_gpu_identify.py) and bench.py's param_error_vs_fp64.
+
+Data: the seeded ground truth of the workload (identification_prior(), ground_truth(seed=1)) and
+256 segments of horizon 20 cut from its trajectories with per-joint sinusoidal PD targets
+(amplitude U[0.1, 0.4] rad, frequency U[0.5, 3] Hz) plus measurement noise (sigma_q 0.01 rad,
+sigma_v 0.05 rad/s) on the joint channels (spi_amd.data.synthetic_segments, seed 2), every
+trajectory simulated by the FP64 oracle.
+
+Minimiser: FP64 Levenberg-Marquardt (spio.gauss_newton_fp64, exact forward-mode Jacobians of
+the oracle's discrete rollout) of the tracking cost mean_s sum_t sum_i w_i (x_i - target_i)^2
+with the workload's weights (10, 5, 1, 1, 1, 0.05 for pos, quat, linvel, angvel, q, qd), over the
+22 identified rows inside the prior's bounds; once started from the truth and once from the
+prior mean, so that the minimiser is shown not to depend on where the search starts (the role
+of the reference's argmin in scripts/mass_opt.py:136-169 and scripts/mass_landscape.py:161-171).
+
+Per-parameter asymptotic standard error of the minimiser (Gaussian residuals: the Cramer-Rao
+bound of the nonlinear least-squares estimate): cov = s^2 (J^T J)^-1, s^2 = r.r / (N - P),
+with r and J the weighted residuals and their Jacobian at the minimiser.
+
+Run from the repository root: python tests/golden/make_c1_fixture.py  (about 5 minutes on 8 cores)
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "spi-active_amd"), os.path.join(ROOT, "oracle")]
+
+import spio  # noqa: E402
+from spi_amd.agents.sysid import ground_truth, identification_prior  # noqa: E402
+from spi_amd.data import synthetic_segments  # noqa: E402
+from spi_amd.model import load_model  # noqa: E402
+from spi_amd.params import PARAM_NAMES, SimParams  # noqa: E402
+
+N_SEG, HORIZON = 256, 20
+TRUTH_SEED, DATA_SEED = 1, 2
+WEIGHTS = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
+GROUP = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
+OUT = os.path.join(HERE, "c1_fp64_minimiser.npz")
+
+
+def main():
+    t0 = time.time()
+    model, sim = load_model(), SimParams()
+    o = spio.Oracle(model, sim)
+    prior = identification_prior()
+    truth = ground_truth(prior, seed=TRUTH_SEED)
+
+    def simulate(rows, init, acts):
+        out = []
+        for r, i0, a in zip(rows, init, acts):
+            traj, rc = o.simulate(np.asarray(r, np.float64), i0, a)
+            if rc != 0:
+                raise RuntimeError("ground-truth trajectory diverged")
+            out.append(traj)
+        return np.stack(out)
+
+    segs = synthetic_segments(simulate, truth, sim, n_seg=N_SEG, horizon=HORIZON, seed=DATA_SEED)
+    print(f"data: {time.time() - t0:.1f} s", flush=True)
+    rows = np.arange(22)
+    seg_w = np.full(N_SEG, 1.0 / N_SEG)
+    fits = {}
+    for name, start in (("truth", truth), ("prior_mean", prior.mean)):
+        t = time.time()
+        th, cost, hist = spio.gauss_newton_fp64(o, start, rows, segs, WEIGHTS, seg_w, prior.lo, prior.hi,
+                                                iterations=40, lam0=1e-3, tol=1e-12)
+        fits[name] = (th, cost, np.asarray(hist))
+        print(f"LM from {name}: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
+              f"({time.time() - t:.0f} s)", flush=True)
+    th = fits["truth"][0]
+    y, J, rc = o.traj_sens_batch(th, rows, segs.init, segs.actions, n_rows=37)
+    sw = np.sqrt(WEIGHTS[GROUP][None, :] * seg_w[:, None])
+    r = (sw[:, None, :] * (y - segs.targets.astype(np.float64))).ravel()
+    Jw = (sw[:, None, :, None] * J).reshape(-1, len(rows))
+    N, P = r.size, len(rows)
+    s2 = float(r @ r) / (N - P)
+    cov = s2 * np.linalg.inv(Jw.T @ Jw)
+    se = np.zeros(25)
+    se[rows] = np.sqrt(np.diag(cov))
+    se_rel = se / np.maximum(np.abs(th), 1e-300)
+    g = Jw.T @ r
+    print("param            truth          fp64 minimiser   from prior mean  se_rel")
+    for i in rows:
+        print(f"{PARAM_NAMES[i]:14s} {truth[i]: .8e} {th[i]: .8e} {fits['prior_mean'][0][i]: .8e} {se_rel[i]:.3e}")
+    print(f"gradient norm at the minimiser {np.linalg.norm(g):.3e}, rc {rc}")
+    np.savez_compressed(
+        OUT, segments=segs.packed(), truth=truth, theta_fp64=th, theta_fp64_from_prior=fits["prior_mean"][0],
+        cost_fp64=fits["truth"][1], cost_fp64_from_prior=fits["prior_mean"][1], history=fits["truth"][2],
+        history_from_prior=fits["prior_mean"][2], se=se, se_rel=se_rel, grad_at_minimiser=g, rows=rows,
+        weights=WEIGHTS, n_seg=N_SEG, horizon=HORIZON, truth_seed=TRUTH_SEED, data_seed=DATA_SEED)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {time.time() - t0:.0f} s)")
+
+
+if __name__ == "__main__":
+    main()

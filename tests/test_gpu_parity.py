@@ -394,39 +394,65 @@ def test_cem_update_matches_numpy(gpu):
     assert np.array_equal(best.cpu().numpy(), theta[:, order[0]])
 
 
-@pytest.mark.parametrize("ksync", [0, 3])
-def test_fim_rollout_matches_oracle(gpu, model, sim, theta0, ksync):
-    """Fused FD-FIM objective (state-copy ksync, or none) vs the oracle stepping main and aux envs
-    and applying the reference formula per step: relative 1e-3."""
+def _fim_oracle_group(o, sim, par, init, acts, delta, ksync, mode):
+    """The reference's FD-FIM sum over T command steps for one group, stepped with the FP64
+    oracle's torque law and physics step.  Command step t has the reference's step_idx t + 2
+    (reset_all's zero-action step is step_idx 1, base_task.py:90-100); after it the aux envs
+    sync to the main env when step_idx % ksync == 1 (active_sysid_openloop.py:246-252).
+    mode "state" copies the whole state; mode "reference" only rewrites the aux envs' joint
+    views (:316-330 via _reset_robot_states_callback), which the next step's first PD torque
+    reads and the simulator never sees."""
+    G = init.shape[0]
+    st = [(e[:13].astype(np.float64), e[13:25].astype(np.float64), e[25:37].astype(np.float64)) for e in init]
+    view = None
+    tot = 0.0
+    for t in range(acts.shape[0]):
+        for k in range(G):
+            root, q, qd = st[k]
+            for d in range(sim.decimation):
+                if view is not None and k > 0 and d == 0:
+                    tau = o.control_torque(acts[t], view[0], view[1], par[k])
+                else:
+                    tau = o.control_torque(acts[t], q, qd, par[k])
+                root, q, qd = o.physics_step(root, q, qd, tau, par[k])
+            st[k] = (root, q, qd)
+        view = None
+        x0 = np.concatenate([st[0][0], st[0][1]])
+        for k in range(1, G):
+            X = (x0 - np.concatenate([st[k][0], st[k][1]])) / delta
+            tot += float(X @ X)
+        if ksync and (ksync == 1 or (t + 2) % ksync == 1):
+            if mode == "state":
+                st = [st[0]] + [tuple(a.copy() for a in st[0]) for _ in range(1, G)]
+            else:
+                view = (st[0][1].copy(), st[0][2].copy())
+    return tot
+
+
+@pytest.mark.parametrize("ksync,mode", [(0, "state"), (3, "state"), (5, "reference"), (1, "reference")])
+def test_fim_rollout_matches_oracle(gpu, model, sim, theta0, ksync, mode):
+    """Fused FD-FIM objective vs the oracle stepping main and aux envs and applying the
+    reference formula per step, with the reference's sync schedule (step_idx counted from the
+    reset step: with ksync 5 the syncs follow command steps 5 and 10) in both sync modes, and
+    with syncs disabled: relative 1e-3.  The aux envs start from perturbed states so that a
+    sync changes the result."""
     import torch
-    import spio
     from spi_amd.agents.active import FDFIMObjective
     from spi_amd.data import sinusoid_actions, standing_state
     o, s = _oracle(model, sim)
     eng = _engine(s)
     rng = np.random.default_rng(30)
     M, T, delta = 5, 12, 0.1
-    obj = FDFIMObjective(eng, theta0, ["mass", "comx"], delta=delta, ksync_steps=max(ksync, 1),
-                         ksync_mode="state")
+    obj = FDFIMObjective(eng, theta0, ["mass", "comx"], delta=delta, ksync_steps=max(ksync, 1), ksync_mode=mode)
     if ksync == 0:
         obj.ksync = 0
     settled, _ = o.simulate(theta0, standing_state(), np.zeros((40, 12), np.float32))
-    init = settled[-1]
-    acts = sinusoid_actions(rng, M, T, 0.02, 0.25)
-    g = obj(torch.from_numpy(acts).to(gpu), torch.from_numpy(init.copy()).to(gpu), mean=False).cpu().numpy()
-    par = obj.params(M).cpu().numpy().T.astype(np.float64)
     G = 3
+    init = np.repeat(settled[-1][None], M * G, axis=0).astype(np.float32)
+    init[:, 13:37] += rng.normal(0, 0.02, (M * G, 24)).astype(np.float32)
+    acts = sinusoid_actions(rng, M, T, 0.02, 0.25)
+    g = obj(torch.from_numpy(acts).to(gpu), torch.from_numpy(init).to(gpu), mean=False).cpu().numpy()
+    par = obj.params(M).cpu().numpy().T.astype(np.float64)
     for m in range(M):
-        states = [init.astype(np.float64).copy() for _ in range(G)]
-        tot = 0.0
-        for t in range(T):
-            for k in range(G):
-                out, _ = o.simulate(par[m * G + k], states[k].astype(np.float32), acts[m, t:t + 1])
-                states[k] = out[0].astype(np.float64)
-            for k in range(1, G):
-                X = (states[0][:25] - states[k][:25]) / delta
-                tot += float(X @ X)
-            if ksync and (t + 1) % ksync == 1:
-                for k in range(1, G):
-                    states[k] = states[0].copy()
+        tot = _fim_oracle_group(o, s, par[m * G:(m + 1) * G], init[m * G:(m + 1) * G], acts[m], delta, ksync, mode)
         assert abs(g[m] - tot) <= 1e-3 * tot + 1e-3, (m, g[m], tot)

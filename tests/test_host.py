@@ -47,7 +47,8 @@ def test_ctypes_structs_match_c_layout():
                   sizeof(SpiCostArgs), offsetof(SpiCostArgs, w), sizeof(SpiBatchProblem),
                   offsetof(SpiBatchProblem, seed), offsetof(SpiBatchProblem, mean),
                   offsetof(SpiBatchProblem, best_cost), sizeof(SpiTangentSpec), offsetof(SpiTangentSpec, param));
-      std::printf("%zu %zu %zu\n", sizeof(SpiCommandSpec), offsetof(SpiCommandSpec, lo), offsetof(SpiCommandSpec, n_bezier));
+      std::printf("%zu %zu %zu ", sizeof(SpiCommandSpec), offsetof(SpiCommandSpec, lo), offsetof(SpiCommandSpec, n_bezier));
+      std::printf("%zu %zu\n", offsetof(SpiCostArgs, seg_gains), offsetof(SpiBatchProblem, cost));
     }
     """
     with tempfile.TemporaryDirectory() as d:
@@ -61,8 +62,27 @@ def test_ctypes_structs_match_c_layout():
               SpiSimArgs.state_limit.offset, ctypes.sizeof(SpiCostArgs), SpiCostArgs.w.offset,
               ctypes.sizeof(SpiBatchProblem), SpiBatchProblem.seed.offset, SpiBatchProblem.mean.offset,
               SpiBatchProblem.best_cost.offset, ctypes.sizeof(SpiTangentSpec), SpiTangentSpec.param.offset,
-              ctypes.sizeof(SpiCommandSpec), SpiCommandSpec.lo.offset, SpiCommandSpec.n_bezier.offset]
+              ctypes.sizeof(SpiCommandSpec), SpiCommandSpec.lo.offset, SpiCommandSpec.n_bezier.offset,
+              SpiCostArgs.seg_gains.offset, SpiBatchProblem.cost.offset]
     assert got == expect
+
+
+def test_segment_gains_known_answer():
+    """Recorded gains per segment: each file's own, or the first window of each chunk of
+    batch_size windows (scripts/eval.py:192-201, 245-250).  Files A (3 windows, kp 20) and
+    B (4 windows, kp 25); chunks of 2: [A A][A B][B B][B] -> kp 20 20 20 20 25 25 25."""
+    from spi_amd.data import segment_gains
+    kp = np.array([20, 20, 20, 25, 25, 25, 25], np.float32)
+    kd = np.array([0.5, 0.5, 0.5, 0.6, 0.6, 0.6, 0.6], np.float32)
+    own = segment_gains(kp, kd)
+    assert own.dtype == np.float32 and own.shape == (7, 2)
+    assert np.array_equal(own[:, 0], kp) and np.array_equal(own[:, 1], kd)
+    ch = segment_gains(np.repeat(kp[:, None], 12, 1), np.repeat(kd[:, None], 12, 1), batch_size=2)
+    assert np.array_equal(ch[:, 0], [20, 20, 20, 20, 25, 25, 25])
+    assert np.array_equal(ch[:, 1], np.array([0.5, 0.5, 0.5, 0.5, 0.6, 0.6, 0.6], np.float32))
+    assert np.array_equal(segment_gains(kp, kd, batch_size=100)[:, 0], np.full(7, 20.0))
+    with pytest.raises(ValueError):
+        segment_gains(kp, kd[:3])
 
 
 def test_reference_segment_weights_known_answer():
@@ -164,7 +184,8 @@ def test_problem_table_packing_matches_c_layout():
     for k in range(5):
         H, n, n_seg = int(rng.choice([5, 20])), int(rng.choice([512, 8192])), int(rng.integers(1, 40))
         specs.append(SimpleNamespace(segments=torch.empty((n_seg, 37 + 49 * H)), horizon=H, n_samples=n,
-                                     elite_frac=0.1, alpha=0.25 * k / 5, seed=1000 + k))
+                                     elite_frac=0.1, alpha=0.25 * k / 5, seed=1000 + k,
+                                     seg_gains=torch.empty((n_seg, 2)) if k % 2 else None))
         bufs.append(SimpleNamespace(ptrs=tuple(int(x) for x in rng.integers(1, 2**47, size=12))))
     ca = cost_args(0, [10, 5, 1, 1, 1, 0.05])
     table, prefix, n_tasks = pack_problem_table(specs, bufs, [0, 3, 1, 7, 2], ca, 128, "cpu")
@@ -175,6 +196,7 @@ def test_problem_table_packing_matches_c_layout():
         (p.segments, p.seg_weight, p.mean, p.std, p.lo, p.hi, p.min_std, p.theta, p.cost_seg, p.cost_out, p.best,
          p.best_cost) = b.ptrs
         p.n_seg, p.horizon, p.cost, p.n_samples = s.segments.shape[0], s.horizon, ca, s.n_samples
+        p.cost.seg_gains = None if s.seg_gains is None else s.seg_gains.data_ptr()
         p.n_elite = max(1, int(math.ceil(s.elite_frac * s.n_samples - 1e-9)))
         p.alpha, p.iteration, p.seed = s.alpha, [0, 3, 1, 7, 2][i], s.seed
         assert prefix[i].item() == tot

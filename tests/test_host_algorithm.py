@@ -76,3 +76,46 @@ def test_host_rollout_matches_oracle(host_lib, model, sim, theta0):
         host_lib.spi_host_control_step(ctypes.byref(args), _p(par), _p(st), _p(_f(acts[t])))
         assert np.abs(st[:7] - ref[t, :7]).max() < 1e-4
         assert np.abs(st[13:25] - ref[t, 13:25]).max() < 1e-3
+
+
+def test_cpu_baseline_rollout_cost_matches_oracle(model, sim, theta0):
+    """The host-CPU baseline (spi_dyn.h built for x86 with OpenMP, tests/host_dyn/Makefile) that
+    bench.py times: its tracking costs match the FP64 oracle (2e-3 relative), like the GPU
+    kernel's, on 6 candidates x 4 noisy segments x H 5, and do not depend on the thread count."""
+    import spio
+    from spi_amd import _lib
+    from spi_amd.data import synthetic_segments
+    from spi_amd.rollout import DEFAULT_TRACKING_WEIGHTS
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "tests", "host_dyn"), "-s"], capture_output=True, text=True)
+    if r.returncode != 0:
+        pytest.skip("host baseline build unavailable: " + r.stderr[-300:])
+    L = ctypes.CDLL(os.path.join(ROOT, "tests", "host_dyn", "lib", "libspi_rollout_cpu.so"))
+    o = spio.Oracle(model, sim)
+
+    def simulate(rows, init, acts):
+        return np.stack([o.simulate(np.asarray(rw, np.float64), i0, a)[0] for rw, i0, a in zip(rows, init, acts)])
+
+    H, n_seg, n_cand = 5, 4, 6
+    segs = synthetic_segments(simulate, theta0, sim, n_seg, H, seed=4)
+    rng = np.random.default_rng(3)
+    rows = np.repeat(theta0[None], n_cand, axis=0)
+    rows[1:, 0] *= rng.uniform(0.8, 1.2, n_cand - 1)
+    rows[1:, 10:18] *= rng.uniform(0.9, 1.1, (n_cand - 1, 8))
+    rows = rows.astype(np.float32)
+    par = _f(rows.T)
+    seg = _f(segs.packed())
+    a = _lib.sim_args(sim)
+    c = _lib.cost_args(0, DEFAULT_TRACKING_WEIGHTS)
+    outs = []
+    for threads in (1, 3):
+        out = np.zeros((n_seg, n_cand), np.float32)
+        used = L.spi_cpu_rollout_cost(ctypes.byref(a), ctypes.byref(c), _p(par), ctypes.c_int(n_cand),
+                                      ctypes.c_int(n_cand), _p(seg), ctypes.c_int(n_seg), ctypes.c_int(H), _p(out),
+                                      ctypes.c_int(threads))
+        assert used == threads
+        outs.append(out)
+    assert np.array_equal(outs[0], outs[1])
+    ref = o.rollout_cost(rows.astype(np.float64), segs.init, segs.actions, segs.targets, 0,
+                         np.asarray(DEFAULT_TRACKING_WEIGHTS, np.float64)).T
+    err = np.abs(outs[0] - ref)
+    assert np.all(err <= 2e-3 * ref + 1e-4), err.max()
