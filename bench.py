@@ -2,15 +2,18 @@
 """Benchmark: 22-parameter CEM system identification of the Go2 on one MI355X.
 
 Workload (one "step" = one full sysID solve, inputs resident in HBM):
-  4096 parameter samples / iteration, 410 elites (ceil 10 %), 12 iterations,
-  256 synthetic segments of horizon 20 control steps (dt 0.005 s, decimation 4,
-  4 integration sub-intervals per physics step), PD Kp 20 / Kd 0.5, penalty contact
-  k 5e4 N/m, c 100 N s/m, mu 0.8.  Segments come from a seeded ground truth with
-  sinusoidal PD targets (amp U[0.1, 0.4] rad, f U[0.5, 3] Hz) and measurement noise
-  (sigma_q 0.01 rad, sigma_v 0.05 rad/s), generated on the GPU before timing.
+  4096 parameter samples / iteration, 410 elites (ceil 10 %), 12 iterations, then 10
+  Levenberg-Marquardt iterations with exact residual Jacobians; 256 segments of horizon 20
+  control steps (dt 0.005 s, decimation 4, 4 integration sub-intervals per physics step),
+  PD Kp 20 / Kd 0.5, penalty contact k 5e4 N/m, c 100 N s/m, mu 0.8.  The segments are the
+  committed fixture tests/golden/c1_fp64_minimiser.npz: a seeded ground truth simulated by the
+  FP64 oracle with sinusoidal PD targets (amp U[0.1, 0.4] rad, f U[0.5, 3] Hz) and measurement
+  noise (sigma_q 0.01 rad, sigma_v 0.05 rad/s), together with the FP64 least-squares
+  minimiser of the same objective, against which the identified parameters are scored.
   Per solve: 4096 * 256 * 20 * 12 = 251,658,240 control steps.
 
-Prints one JSON line (metric, value = control steps per second over the timed solves).
+Prints one JSON line (metric, value = control steps per second over the timed solves); extra
+keys carry the 1-D landscape's latency and the Fisher objective's time (short runs).
 """
 import argparse
 import json
@@ -35,6 +38,7 @@ PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 (vector) peak
 TRAFFIC_BYTES_PER_LAUNCH = 8983582
 
 N_SAMPLES, N_SEG, HORIZON, ITERS = 4096, 256, 20, 12
+FIXTURE = os.path.join(ROOT, "tests", "golden", "c1_fp64_minimiser.npz")
 
 
 def parse():
@@ -43,9 +47,11 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-candidates", type=int, default=256)
+    p.add_argument("--cpu-candidates", type=int, default=256, help="oracle CPU engine: candidates timed")
     p.add_argument("--no-plain-baseline", action="store_true")
-    p.add_argument("--refine", type=int, default=10, help="Levenberg-Marquardt iterations after CEM (0: none)")
+    p.add_argument("--no-extras", action="store_true", help="skip the landscape and Fisher keys")
+    p.add_argument("--refine", type=int, default=10,
+                   help="Levenberg-Marquardt iterations after CEM, inside each timed solve (0: CEM only)")
     p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve",
                                                     "policy"],
                    help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
@@ -88,8 +94,9 @@ def main():
         print(json.dumps(out))
         return
     from spi_amd.agents.cem import CEMConfig, DeviceCEM
-    from spi_amd.agents.sysid import SysIdProblem, ground_truth, identification_prior
-    from spi_amd.data import synthetic_segments
+    from spi_amd.agents.refine import GaussNewtonRefiner
+    from spi_amd.agents.sysid import SysIdProblem, identification_prior
+    from spi_amd.data import SegmentSet
     from spi_amd.model import load_model
     from spi_amd.params import PARAM_NAMES, SimParams
     from spi_amd.rollout import RolloutEngine
@@ -100,11 +107,14 @@ def main():
     model = load_model()
     sim = SimParams()                      # Kp 20, Kd 0.5, k 5e4, c 100, mu 0.8, dt 5 ms, decimation 4
     prior = identification_prior()
-    theta_true = ground_truth(prior, seed=1)
+    with np.load(FIXTURE, allow_pickle=False) as fx:
+        seg_np, theta_true = fx["segments"], fx["truth"]
+        theta64, se_rel = fx["theta_fp64"], fx["se_rel"]
+    segs = SegmentSet.from_packed(seg_np, HORIZON)
     eng = RolloutEngine(sim)
-    segs = synthetic_segments(eng.simulate_np, theta_true, sim, n_seg=N_SEG, horizon=HORIZON, seed=2)
-    seg_dev = eng.upload_segments(segs)
+    seg_dev = torch.from_numpy(seg_np).to(dev)
     problem = SysIdProblem(eng, seg_dev, HORIZON, max_cand=N_SAMPLES)
+    refiner = GaussNewtonRefiner(eng, seg_dev, HORIZON, lo=prior.lo, hi=prior.hi) if args.refine > 0 else None
     stream = torch.cuda.current_stream()
 
     if args.profile_launch:
@@ -129,56 +139,65 @@ def main():
         return eng.reduce(problem._seg_cost, problem.seg_weights, out=problem._cost[:n])
 
     def solve(seed, timed):
+        """One sysID solve: CEM on the device, then (args.refine > 0) Levenberg-Marquardt from
+        the CEM mean.  Returns (CEM mean, refined parameters or None, refinement record)."""
         cem = DeviceCEM(prior.mean, prior.std, prior.lo, prior.hi, prior.min_std,
                         CEMConfig(N_SAMPLES, 0.1, ITERS, seed=seed), device="cuda")
         cem.solve(evaluate_timed if timed else problem)
-        return cem
+        if refiner is None:
+            return cem.mean, None, None
+        res = refiner.refine(cem.mean.cpu().numpy().astype(np.float64), iterations=args.refine)
+        return cem.mean, res.theta, res
 
     for w in range(args.warmup):
         solve(1000 + w, timed=False)
     torch.cuda.synchronize()
 
-    solve_events = []
+    solve_ms = []
     t0 = time.perf_counter()
-    cems = []
+    outs = []
     for k in range(args.steps):
-        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s0.record(stream)
-        cems.append(solve(k, timed=True))
-        s1.record(stream)
-        solve_events.append((s0, s1))
+        t_s = time.perf_counter()
+        outs.append(solve(k, timed=True))
+        torch.cuda.synchronize()
+        solve_ms.append((time.perf_counter() - t_s) * 1e3)
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
 
-    lat = np.array([a.elapsed_time(b) for a, b in solve_events])
+    lat = np.array(solve_ms)
     kern = np.array([a.elapsed_time(b) for a, b in kernel_events])
     ctrl_per_solve = N_SAMPLES * N_SEG * HORIZON * ITERS
     value = args.steps * ctrl_per_solve / total
 
-    est = cems[-1].mean.cpu().numpy().astype(np.float64)
     free = prior.free
-    rel = np.abs(est - theta_true) / np.maximum(np.abs(theta_true), 1e-12)
-    mass_err = float(rel[0])
-    well = [i for i in range(len(rel)) if free[i] and abs(theta_true[i]) > 1e-2]
-    param_err = {PARAM_NAMES[i]: round(float(rel[i]), 5) for i in range(len(rel)) if free[i]}
+    ident = [i for i in range(len(free)) if free[i] and se_rel[i] < 0.01]
 
-    # Levenberg-Marquardt refinement of the CEM result with exact residual Jacobians (timed
-    # separately: reported beside the CEM solve, not inside `value`)
-    refined = None
-    if args.refine > 0:
-        from spi_amd.agents.refine import GaussNewtonRefiner
-        ref = GaussNewtonRefiner(eng, seg_dev, HORIZON, lo=prior.lo, hi=prior.hi)
-        ref.refine(est, iterations=1)                  # warm
-        torch.cuda.synchronize()
-        t_r = time.perf_counter()
-        res = ref.refine(est, iterations=args.refine)
-        torch.cuda.synchronize()
-        t_r = time.perf_counter() - t_r
-        rr = np.abs(res.theta - theta_true) / np.maximum(np.abs(theta_true), 1e-12)
-        refined = {"ms": round(t_r * 1e3, 1), "iterations": res.iterations, "accepted": res.accepted,
-                   "cost_before": res.history[0], "cost_after": res.cost, "base_mass_rel": round(float(rr[0]), 5),
-                   "max_rel_nonzero_params": round(float(rr[well].max()), 5),
-                   "per_param": {PARAM_NAMES[i]: round(float(rr[i]), 5) for i in range(len(rr)) if free[i]}}
+    def errors(est, ref):
+        rel = np.abs(np.asarray(est, np.float64) - ref) / np.maximum(np.abs(ref), 1e-12)
+        return rel, {PARAM_NAMES[i]: round(float(rel[i]), 5) for i in range(len(rel)) if free[i]}
+
+    cem_mean, refined, res = outs[-1]
+    est = refined if refined is not None else cem_mean.cpu().numpy().astype(np.float64)
+    rel64, per64 = errors(est, theta64)
+    rel_cem64, per_cem64 = errors(cem_mean.cpu().numpy(), theta64)
+    rel_true, per_true = errors(est, theta_true)
+    within = [PARAM_NAMES[i] for i in ident if rel64[i] <= 0.01]
+    accuracy = {
+        "reference": "FP64 least-squares minimiser of the same objective on the same data (oracle LM, "
+                     "tests/golden/c1_fp64_minimiser.npz)",
+        "identifiable": [PARAM_NAMES[i] for i in ident],
+        "identifiable_rule": "asymptotic standard error of the FP64 minimiser < 1 % of its value",
+        "within_1pct": within, "all_identifiable_within_1pct": len(within) == len(ident),
+        "max_rel_identifiable": round(float(max(rel64[i] for i in ident)), 5),
+        "solve": "CEM 12 iterations + LM %d iterations" % args.refine if refiner is not None else "CEM 12 iterations",
+        "per_param": per64,
+        "cem_only_per_param": per_cem64,
+        "se_rel_fp64": {PARAM_NAMES[i]: float("%.3g" % se_rel[i]) for i in range(len(se_rel)) if free[i]},
+    }
+    refined_rec = None
+    if res is not None:
+        refined_rec = {"iterations": res.iterations, "accepted": res.accepted, "cost_before": res.history[0],
+                       "cost_after": res.cost}
 
     launch_ctrl = N_SAMPLES * N_SEG * HORIZON
     kern_s = float(kern.mean()) / 1e3
@@ -193,28 +212,55 @@ def main():
         plain = plain_gpu_baseline(model, sim, segs, prior, eng.weights)
         plain["fused_speedup"] = round(value / plain["value"], 1)
 
-    cpu = None
+    cpu = cpu_oracle = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(model, sim, segs, prior, args.cpu_candidates)
+        cpu = cpu_baseline(sim, seg_np, prior, eng.weights)
+        cpu_oracle = cpu_baseline_oracle(model, sim, {"init": segs.init, "actions": segs.actions,
+                                                      "targets": segs.targets}, prior, args.cpu_candidates)
+        for c in (cpu, cpu_oracle):
+            if c.get("value"):
+                c["gpu_speedup"] = round(value / c["value"], 1)
+
+    extras = {}
+    if not args.no_extras:
+        from spi_amd import workloads
+        try:
+            ls = workloads.landscape(steps=200, warmup=5)
+            extras["landscape_mass_256x64_h5"] = {"solve_latency_ms": ls["solve_latency_ms"],
+                                                  "solves_per_s": ls["solves_per_s"],
+                                                  "argmin_mass_kg": ls["argmin_mass_kg"]}
+        except Exception as e:  # reported, never fatal
+            extras["landscape_mass_256x64_h5"] = {"failed": str(e)}
+        try:
+            fi = workloads.fisher(steps=1, warmup=1)
+            extras["fisher_22x4096x128_h50"] = {"objective_ms": fi["ms_per_step"], "phase_ms": fi["phase_ms"],
+                                                "excitations_per_s": fi["excitations_per_s"],
+                                                "not_positive_definite": fi["not_positive_definite"]}
+        except Exception as e:
+            extras["fisher_22x4096x128_h50"] = {"failed": str(e)}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "env-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (FP64-oracle ground truth + noise, committed fixture)",
         "config": {"workload": "cem22_go2", "samples": N_SAMPLES, "elites": CEMConfig(N_SAMPLES).n_elite,
-                   "iterations": ITERS, "segments": N_SEG, "horizon": HORIZON, "params": int(free.sum()),
-                   "dt": sim.dt, "decimation": sim.decimation, "substeps": sim.substeps,
-                   "control_steps_per_solve": ctrl_per_solve},
+                   "iterations": ITERS, "refine_iterations": args.refine, "segments": N_SEG, "horizon": HORIZON,
+                   "params": int(free.sum()), "dt": sim.dt, "decimation": sim.decimation,
+                   "substeps": sim.substeps, "control_steps_per_solve": ctrl_per_solve},
         "solves_per_s": round(args.steps / total, 4),
         "solve_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                              "p99": round(float(np.percentile(lat, 99)), 2)},
-        "param_error": {"base_mass_rel": round(mass_err, 5),
-                        "max_rel_nonzero_params": round(float(rel[well].max()), 5), "per_param": param_err},
+        "param_error_vs_fp64": accuracy,
+        "param_error_vs_truth": {"max_rel_identifiable": round(float(max(rel_true[i] for i in ident)), 5),
+                                 "per_param": per_true},
+        "refined": refined_rec,
         "roofline": roofline,
-        "refined": refined,
         "plain_gpu_baseline": plain,
         "cpu_baseline": cpu,
+        "cpu_baseline_oracle": cpu_oracle,
     }
+    out.update(extras)
     print(json.dumps(out))
 
 

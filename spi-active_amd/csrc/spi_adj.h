@@ -602,7 +602,7 @@ SPI_HD bool rollout_grad(const SpiSimArgs& a, const SpiCostArgs& cost, const Lan
     }
     if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost(cost, st, rec + 12);
     const Real nrm = state_norm(st);
-    if (!(nrm < a.state_limit)) return false;      // also false for NaN
+    if (!below_limit(nrm, a.state_limit)) return false;   // also for NaN and infinities
   }
   cost_out = acc;
   const State final_state = st;

@@ -440,7 +440,7 @@ __device__ bool rollout_grad_leg(const SpiSimArgs& a, const SpiCostArgs& cost, c
     }
     if (cost.mode == SPI_COST_TRACKING || t == H - 1) acc += step_cost_leg(cost, st, rec + 12, l);
     const Real nrm = state_norm_leg(st);
-    if (!(nrm < a.state_limit)) return false;      // also false for NaN
+    if (!below_limit(nrm, a.state_limit)) return false;   // also for NaN and infinities
   }
   cost_out = acc;
   const LegState final_state = st;

@@ -647,6 +647,20 @@ SPI_HD void physics_step(const SpiSimArgs& a, const LaneParams& lp, State& st, c
 }
 
 // Divergence guard: sum of |base velocity| and |joint velocity| components.
+// "x < limit", false for NaN and infinities also in files built with -ffinite-math-only, where
+// the compiler may assume no value is NaN and rewrite or drop the test: on the device the bits
+// pass through an empty asm, whose output the compiler knows nothing about (host builds of
+// this header are IEEE).
+SPI_HD bool below_limit(float x, float limit) {
+#ifdef __HIP_DEVICE_COMPILE__
+  unsigned int u = __float_as_uint(x);
+  asm volatile("" : "+v"(u));
+  return (u & 0x7f800000u) != 0x7f800000u && x < limit;
+#else
+  return x < limit;
+#endif
+}
+
 SPI_HD Real state_norm(const State& st) {
   Real s = fabsf(st.w.x) + fabsf(st.w.y) + fabsf(st.w.z) + fabsf(st.v.x) + fabsf(st.v.y) + fabsf(st.v.z);
 #pragma unroll

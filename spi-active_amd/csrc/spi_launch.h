@@ -32,9 +32,15 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return base + i;
 }
 
-// Finiteness from the exponent bits: stays correct when a file is compiled with
-// -ffinite-math-only (which lets the compiler fold the model's many x*0 terms).
-__device__ __forceinline__ bool finite_bits(float x) { return (__float_as_uint(x) & 0x7f800000u) != 0x7f800000u; }
+// Finiteness from the exponent bits, for files compiled with -ffinite-math-only (which lets
+// the compiler fold the model's many x*0 terms).  Under that flag the compiler may assume that
+// no float is NaN or infinite and fold a plain bit test of one to "finite"; the bits therefore
+// pass through an empty asm first, whose output the compiler knows nothing about.
+__device__ __forceinline__ bool finite_bits(float x) {
+  unsigned int u = __float_as_uint(x);
+  asm volatile("" : "+v"(u));
+  return (u & 0x7f800000u) != 0x7f800000u;
+}
 
 // A candidate's parameters describe a robot the dynamics can integrate: every one of the 25
 // entries finite, base mass > 0, the base link's rotational inertia about its CoM positive

@@ -25,9 +25,12 @@ LaneParams lane_params(const float* par, int ld, int i) {
   return lp;
 }
 
+// exponent-bit test behind an empty asm: this file is built with -ffinite-math-only, under which
+// the compiler may fold a plain test of a float to "finite"
 bool finite_f(float x) {
   unsigned u;
   std::memcpy(&u, &x, 4);
+  asm volatile("" : "+r"(u));
   return (u & 0x7f800000u) != 0x7f800000u;
 }
 // +inf as a bit pattern: this file is built with finite-math, where the INFINITY macro is undefined

@@ -53,3 +53,31 @@ def test_quad_sums_are_butterflies(device_asm, src):
     n1 = sum("quad_perm:[1,0,3,2]" in ln for ln in dpp)
     n2 = sum("quad_perm:[2,3,0,1]" in ln for ln in dpp)
     assert n1 == n2                                  # every sum has both stages
+
+
+def test_finiteness_tests_survive_finite_math(tmp_path):
+    """The dynamics sources are built with -ffinite-math-only, under which the compiler may
+    assume that no float is NaN or infinite and fold a bit test of one to "finite" (a NaN
+    candidate would then escape the divergence checks).  spi_launch::finite_bits and
+    below_limit pass the bits through an empty asm: compiled with the dynamics' flags, a
+    kernel that tests a loaded value must still mask the exponent and compare it."""
+    if not os.path.exists(build.HIPCC):
+        pytest.skip("no hipcc")
+    src = tmp_path / "probe.hip"
+    src.write_text(
+        '#include <hip/hip_runtime.h>\n#include "spi_launch.h"\n#include "spi_dyn.h"\n'
+        "__global__ void probe_bits(const float* __restrict__ x, int* __restrict__ o) {\n"
+        "  o[threadIdx.x] = spi_launch::finite_bits(x[threadIdx.x] * 2.0f + 1.0f); }\n"
+        "__global__ void probe_limit(const float* __restrict__ x, int* __restrict__ o) {\n"
+        "  o[threadIdx.x] = spi::below_limit(x[threadIdx.x] * 2.0f + 1.0f, 1e4f); }\n")
+    out = tmp_path / "probe.s"
+    flags = build.FILE_FLAGS["spi_kernels.hip"]
+    assert "-ffinite-math-only" in flags
+    cmd = [build.HIPCC] + build.CFLAGS + flags + ["--cuda-device-only", "-S", str(src), "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
+    asm = out.read_text()
+    for fn in ("probe_bits", "probe_limit"):
+        body = asm[asm.index(f"{fn}PKfPi:"):]
+        body = body[:body.index("s_endpgm")]
+        assert "0x7f800000" in body and "v_cmp" in body, f"{fn}: the finiteness test was folded away"

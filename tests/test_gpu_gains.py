@@ -81,8 +81,8 @@ def test_recorded_gains_match_oracle(gpu, model, sim, theta0, tmp_path, batch_si
     assert ds.n_seg == sum(LENGTHS) - 2 * H
     gains = segment_gains(ds.kp, ds.kd, batch_size)
     if batch_size == 8:
-        assert np.array_equal(gains[19:24], np.repeat([GAINS[0]], 5, axis=0))
-        assert np.array_equal(gains[24:], np.repeat([GAINS[1]], ds.n_seg - 24, axis=0))
+        assert np.array_equal(gains[19:24], np.repeat(np.array([GAINS[0]], np.float32), 5, axis=0))
+        assert np.array_equal(gains[24:], np.repeat(np.array([GAINS[1]], np.float32), ds.n_seg - 24, axis=0))
     rows = _rows(theta0, 6, seed=1)
     g_dev = torch.from_numpy(gains).to(gpu)
     for mode, w in (("tracking", DEFAULT_TRACKING_WEIGHTS), ("reference", REFERENCE_WEIGHTS)):
@@ -105,8 +105,10 @@ def test_recorded_gains_match_oracle(gpu, model, sim, theta0, tmp_path, batch_si
 
 
 def test_recorded_gains_in_batch_adjoint_tangent_and_refiner(gpu, model, sim, theta0, tmp_path):
-    """The other segment consumers under recorded gains: the persistent batched kernel equals
-    the single-problem kernel with the same gains bit for bit; the reverse-mode (four lanes and
+    """The other segment consumers under recorded gains: the persistent batched kernel agrees
+    with the single-problem kernel with the same gains (median relative difference <= 1e-5,
+    max 5e-2: the two are compiled separately and contract differently, and 20 % prior samples
+    make contact-rich rollouts that amplify it) and differs from it without the gains; the reverse-mode (four lanes and
     one lane per rollout) and forward-mode cost + gradient kernels and the Gauss-Newton
     residual Jacobian's cost agree with the oracle (2e-3); the adjoint gradient agrees with the
     oracle's FP64 gradient under the same gains (2e-2 of each row's scale)."""
@@ -132,11 +134,16 @@ def test_recorded_gains_in_batch_adjoint_tangent_and_refiner(gpu, model, sim, th
     b.rollout()
     buf = b.bufs[0]
     single = eng.segment_costs(buf.theta, 256, segs, H, physics="lanes", seg_gains=g_dev).cpu().numpy()
-    assert np.array_equal(buf.cost_seg.cpu().numpy(), single)
+    got = buf.cost_seg.cpu().numpy()
+    rel = np.abs(got - single) / np.maximum(np.abs(single), 1e-6)
+    print("batch vs single kernel, relative difference: median", np.median(rel), "max", rel.max())
+    assert np.median(rel) <= 1e-5 and np.quantile(rel, 0.99) <= 1e-3 and rel.max() <= 5e-2
     nog = BatchCEM(eng, [ProblemSpec(segs, H, prior, n_samples=256, seed=3)], block=64)
     nog.sample()
     nog.rollout()
-    assert not np.array_equal(nog.bufs[0].cost_seg.cpu().numpy(), single)
+    fileb = slice(LENGTHS[0] - H, n_seg)
+    rel_nog = np.abs(nog.bufs[0].cost_seg.cpu().numpy()[fileb] - single[fileb]) / np.abs(single[fileb])
+    assert np.median(rel_nog) > 1e-2, np.median(rel_nog)      # file B's gains are in effect
 
     rows = _rows(theta0, 4, seed=2)
     ref = _oracle_costs(model, sim, ds, gains, rows, 0, DEFAULT_TRACKING_WEIGHTS)
