@@ -348,8 +348,10 @@ template <class T> V6<T> base_body_velocity(const State<T>& st) {
 }
 
 // Penalty contact forces, returned as spatial forces in link coordinates (fext[link]).
-// Ground plane z = 0.  Normal: Fn = max(0, k*pen - c*vz); tangential: viscous with slope
-// c_tangent, capped by the Coulomb cone mu*Fn.
+// Ground plane z = 0.  Normal: Fn = k*pen*(1 - tanh(c*vz / (k*pen))), a spring-damper
+// (k*pen - c*vz where |c*vz| << k*pen) whose damping force saturates at the spring force, so
+// 0 <= Fn <= 2*k*pen and Fn -> 0 at the contact onset whatever the approach speed; tangential:
+// viscous with slope c_tangent, capped by the Coulomb cone mu*Fn.
 template <class T> void contact_forces(const Tree<T>& tr, const Kin<T>& k, const SimConfig& cfg,
                                        std::vector<V6<T>>& fext, T* foot_force_world = nullptr) {
   for (auto& f : fext) f = V6<T>{};
@@ -371,8 +373,8 @@ template <class T> void contact_forces(const Tree<T>& tr, const Kin<T>& k, const
       V3<T> wxp = cross(w, pc);
       V3<T> vl{vo[0] + wxp[0], vo[1] + wxp[1], vo[2] + wxp[2]};
       V3<T> vc = mv3(R, vl);
-      T fn = T(cfg.k_contact) * pen - T(cfg.c_contact) * vc[2];
-      if (!s_pos(fn)) fn = T(0);
+      const T kpen = T(cfg.k_contact) * pen;
+      T fn = kpen * (T(1) - s_tanh(T(cfg.c_contact) * vc[2] / kpen));   // damping saturates at k*pen
       T vt2 = vc[0] * vc[0] + vc[1] * vc[1];
       T ft = T(0), sc = T(0);
       if (s_pos(vt2)) {

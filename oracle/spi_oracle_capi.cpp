@@ -319,6 +319,40 @@ int spio_traj_sens_batch(void* hp, const double* par, const int* idx, int P, con
   return bad;
 }
 
+// spio_simulate of one parameter vector from n_traj initial states in parallel (OpenMP):
+// init37 [n_traj][37], actions [n_traj][n_steps][12], states_out [n_traj][n_steps][37] in double
+// (root 13, q 12, qd 12 after each control step).  Returns the number of trajectories that left
+// the finite range.
+int spio_simulate_batch(void* hp, const double* par, const float* init37, const float* actions, int n_traj,
+                        int n_steps, double* states_out, int n_threads) {
+  auto* h = static_cast<Handle<double>*>(hp);
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#else
+  (void)n_threads;
+#endif
+  int bad = 0;
+#pragma omp parallel reduction(+ : bad)
+  {
+    Engine<double> eng = h->eng;
+    eng.apply_base_params(par);
+#pragma omp for schedule(dynamic, 1)
+    for (int i = 0; i < n_traj; ++i) {
+      State<double> st;
+      load37(st, init37 + size_t(i) * 37);
+      for (int t = 0; t < n_steps; ++t) {
+        double a[12];
+        for (int j = 0; j < 12; ++j) a[j] = actions[(size_t(i) * n_steps + t) * 12 + j];
+        eng.control_step(st, a, par);
+        double* o = states_out + (size_t(i) * n_steps + t) * 37;
+        store_state(st, o, o + 13, o + 25);
+      }
+      bad += !finite_state(st);
+    }
+  }
+  return bad;
+}
+
 // FP64 rollout cost of n_cand parameter vectors on n_seg segments with the (candidate, segment)
 // pairs spread over OpenMP threads; same arguments and result as spio_rollout_cost.
 int spio_rollout_cost_par(void* hp, const double* par, int n_cand, const float* seg_init, const float* seg_actions,

@@ -88,6 +88,7 @@ def lib(path: Optional[str] = None):
                                                C.c_int]
             L.spio_rollout_cost_par.argtypes = [vp, dp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, dp,
                                                 C.c_int]
+            L.spio_simulate_batch.argtypes = [vp, dp, fp, fp, C.c_int, C.c_int, dp, C.c_int]
         L.spio_rollout_cost_f32.argtypes = [vp, fp, C.c_int, fp, fp, fp, C.c_int, C.c_int, C.c_int, dp, fp,
                                             C.c_int]
         L.spio_max_threads.restype = C.c_int
@@ -319,6 +320,17 @@ class Oracle:
                                           _fp(actions), n, T, n_rows, _dp(y), _dp(J), int(n_threads))
         return y, J, rc
 
+    def simulate_batch(self, par, init37, actions, n_threads=0):
+        """FP64 states after every control step of many trajectories under one parameter vector,
+        in parallel: [n, T, 37] double, number of trajectories that left the finite range."""
+        par = np.ascontiguousarray(par, dtype=np.float64)
+        init37 = np.ascontiguousarray(init37, dtype=np.float32)
+        actions = np.ascontiguousarray(actions, dtype=np.float32)
+        n, T = actions.shape[0], actions.shape[1]
+        out = np.zeros((n, T, 37))
+        rc = self._L.spio_simulate_batch(self.h, _dp(par), _fp(init37), _fp(actions), n, T, _dp(out), int(n_threads))
+        return out, rc
+
     def rollout_cost_par(self, par, seg_init, seg_actions, seg_targets, cost_mode, weights, n_threads=0):
         """rollout_cost with the (candidate, segment) pairs over threads: [n_cand, n_seg]."""
         par = np.ascontiguousarray(par, dtype=np.float64).reshape(-1, P_COUNT)
@@ -394,7 +406,10 @@ def gauss_newton_fp64(o: "Oracle", theta0, rows, segs, weights, seg_w, lo=None, 
     seg_w = np.asarray(seg_w, np.float64)
 
     def resid(theta, with_jac):
-        y, J, _ = o.traj_sens_batch(theta, rows, segs.init, segs.actions, n_rows=37)   # [n, T, 37(, P)]
+        if with_jac:
+            y, J, _ = o.traj_sens_batch(theta, rows, segs.init, segs.actions, n_rows=37)   # [n, T, 37(, P)]
+        else:
+            y, _ = o.simulate_batch(theta, segs.init, segs.actions)
         sw = np.sqrt(w[grp][None, :] * seg_w[:, None])                                  # [n, 37]
         R = (sw[:, None, :] * (y - segs.targets.astype(np.float64))).ravel()
         return R, ((sw[:, None, :, None] * J).reshape(-1, len(rows)) if with_jac else None)

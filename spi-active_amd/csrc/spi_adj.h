@@ -154,8 +154,11 @@ template <int L> SPI_HD void foot_contact_vjp(const SpiSimArgs& a, const G3& Rc,
   const V3 pcl = pf + down;
   const V3 vl = vc.v + cross(vc.w, pcl);
   const V3 vw = mul(Rc, vl);
-  const Real fn_raw = a.k_contact * pen - a.c_contact * vw.z;
-  const Real fn = fmaxf(fn_raw, 0.0f);
+  // foot_contact's normal force fn = k pen (1 - tanh(u)), u = c vz / (k pen)
+  const Real kpen = a.k_contact * pen;
+  const Real u = a.c_contact * vw.z * spi_rcp(kpen);
+  const Real th = spi_tanh(u);
+  const Real fn = kpen - kpen * th;
   const Real vt2 = vw.x * vw.x + vw.y * vw.y;
   Real sc(0.f), vt(0.f);
   bool cone = false;
@@ -184,11 +187,10 @@ template <int L> SPI_HD void foot_contact_vjp(const SpiSimArgs& a, const G3& Rc,
     vwb.x += vtb * vw.x / vt;
     vwb.y += vtb * vw.y / vt;
   }
-  Real penb(0.f);
-  if (fn_raw > 0.0f) {
-    penb += a.k_contact * fnb;
-    vwb.z += Real(-1.f) * a.c_contact * fnb;
-  }
+  // d fn / d pen = k ((1 - tanh u) + u sech^2 u),  d fn / d vz = -c sech^2 u
+  const Real sech2 = Real(1.f) - th * th;
+  const Real penb = a.k_contact * (Real(1.f) - th + u * sech2) * fnb;
+  vwb.z += Real(-1.f) * a.c_contact * sech2 * fnb;
   // vw = Rc vl
   V3 vlb = mulT(Rc, vwb);
   Rcb = gadd(Rcb, outer(vwb, vl));

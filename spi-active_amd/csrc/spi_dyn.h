@@ -426,6 +426,8 @@ template <int K> SPI_HD void joint_downdate(ABI& IA, F6& pA, Real tau, JointAcc&
 
 // Contact of one foot sphere with the ground plane z = 0.  Inputs: calf pose (world) and
 // calf spatial velocity (calf coords).  Output: contact wrench on the calf in calf coords.
+// Normal force: spring k and damper c, the damping force saturating at the spring force;
+// tangential: viscous with slope c_tangent, capped by the Coulomb cone mu Fn.
 template <int L> SPI_HD F6 foot_contact(const SpiSimArgs& a, const G3& Rc, V3 pc_world, const M6& vc, Real* fout) {
   constexpr CV3 pfc{float(SPI_FOOT_POS[3 * L]), float(SPI_FOOT_POS[3 * L + 1]), float(SPI_FOOT_POS[3 * L + 2])};
   constexpr float rad = float(SPI_FOOT_RADIUS[L]);
@@ -440,7 +442,15 @@ template <int L> SPI_HD F6 foot_contact(const SpiSimArgs& a, const G3& Rc, V3 pc
     V3 pcl = pf + down;
     V3 vl = vc.v + cross(vc.w, pcl);
     V3 vw = mul(Rc, vl);
-    Real fn = fmaxf(a.k_contact * pen - a.c_contact * vw.z, 0.0f);
+    // Spring-damper normal force whose damping force saturates at the spring force:
+    // Fn = k pen (1 - tanh(c vz / (k pen))).  Where |c vz| << k pen it is k pen - c vz, the
+    // spring-damper with stiffness k and damping c; it stays in [0, 2 k pen], so it vanishes at
+    // the contact onset whatever the approach speed and never pulls, and inside the contact it
+    // is smooth in the state and the parameters.  The unsaturated law max(0, k pen - c vz)
+    // jumps by c |vz| at touchdown, which makes the identification objective jump under
+    // arbitrarily small parameter changes.
+    const Real kpen = a.k_contact * pen;
+    const Real fn = kpen - kpen * spi_tanh(a.c_contact * vw.z * spi_rcp(kpen));
     Real vt2 = vw.x * vw.x + vw.y * vw.y;
     Real sc = 0.0f;
     if (vt2 > 0.0f) {

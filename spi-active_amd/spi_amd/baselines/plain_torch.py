@@ -218,7 +218,8 @@ class PlainTorchRollout:
         pcl = self.foot + down
         vl = v[2][..., 3:] + torch.cross(v[2][..., :3], pcl, dim=-1)
         vw = self._mv(Rc, vl)
-        fn = torch.clamp(s.k_contact * pen - s.c_contact * vw[..., 2], min=0.0)
+        kpen = s.k_contact * torch.clamp(pen, min=1e-12)          # masked below where pen <= 0
+        fn = kpen * (1.0 - torch.tanh(s.c_contact * vw[..., 2] / kpen))
         vt = torch.sqrt(vw[..., 0] ** 2 + vw[..., 1] ** 2)
         ft = torch.minimum(s.c_tangent * vt, s.mu * fn)
         scale = torch.where(vt > 0, ft / torch.where(vt > 0, vt, torch.ones_like(vt)), torch.zeros_like(vt))

@@ -189,7 +189,7 @@ def _rollout_case(model, sim, theta0, mode, seed, T, tilt=None, physics="auto"):
     init = np.repeat(settled[-1][None], M * G, axis=0)
     init[:, 7:13] += reset_states(M * G, s.q0, seed=seed, vel_range=0.1)[:, 7:13]
     if tilt is not None:
-        ang = np.pi / 2.5
+        ang = np.pi / 2
         init[tilt * G:(tilt + 1) * G, 3], init[tilt * G:(tilt + 1) * G, 6] = np.sin(ang / 2), np.cos(ang / 2)
     cmds = _commands(rng, M, T)
     fim = roll.evaluate(_dev(cmds), _dev(init)).cpu().numpy()
@@ -218,7 +218,7 @@ def test_policy_rollout_matches_oracle(gpu, model, sim, theta0, mode, physics):
 
 
 def test_policy_rollout_termination(gpu, model, sim, theta0):
-    """Group 1 starts rolled 72 deg: it terminates (the whole group) and terminated steps earn the
+    """Group 1 starts rolled 90 deg: it terminates (the whole group) and terminated steps earn the
     termination reward; the other groups follow the oracle as before."""
     roll, fim, fr, final, done = _rollout_case(model, sim, theta0, "reference", seed=12, T=10, tilt=1)
     assert done[2] and done[3]

@@ -129,33 +129,61 @@ def test_leg_parallel_residual_jacobian_matches(gpu, sim, theta0):
             assert (s4 == 1).all() and (J4 == 0).all() and (J1 == 0).all()
 
 
-def test_cem_identifies_fp64_oracle_minimiser(gpu, sim):
-    """The device CEM (4096 samples, 10 % elites, 12 iterations) on noisy synthetic data
-    (16 segments x H 10, the 22-parameter recipe) against the FP64 oracle's Levenberg-Marquardt
-    minimiser of the same objective, started from the CEM mean: the best-determined parameters
-    (base mass, the four Kp scales) agree to 1 %.  The truth is not the reference point: with
-    noisy initial states the minimiser itself lies away from it."""
+def test_cem_and_refinement_find_fp64_oracle_minimiser(gpu, model, sim):
+    """The device pipeline (CEM: 4096 samples, 10 % elites, 12 iterations; then 10 device
+    Levenberg-Marquardt iterations) on 16 noisy segments x H 10 of the 22-parameter recipe,
+    simulated by the FP64 oracle, against the FP64 oracle's own Levenberg-Marquardt minimiser
+    of the same objective started from the truth (so agreement is not built in): the FP64 run
+    accepts steps and ends at a stationary point (relative gradient <= 1e-3), the device
+    pipeline reaches a cost within 1e-4 of the FP64 minimum, every parameter whose asymptotic
+    standard error at the minimiser is below 1 % agrees to 1 %, and all 22 agree to 2 %."""
     import spio
     from spi_amd.agents.cem import CEMConfig, DeviceCEM
     from spi_amd.agents.fisher import IDENTIFIED, _rows
+    from spi_amd.agents.refine import GaussNewtonRefiner
     from spi_amd.agents.sysid import SysIdProblem, ground_truth, identification_prior
     from spi_amd.data import synthetic_segments
-    from spi_amd.model import load_model
     from spi_amd.params import PARAM_NAMES
     from spi_amd.rollout import RolloutEngine
     eng = RolloutEngine(sim)
     prior = identification_prior()
     truth = ground_truth(prior, seed=1)
     n_seg, H = 16, 10
-    segs = synthetic_segments(eng.simulate_np, truth, sim, n_seg=n_seg, horizon=H, seed=2)
-    problem = SysIdProblem(eng, eng.upload_segments(segs), H, max_cand=4096)
+    o = spio.Oracle(model, sim)
+
+    def simulate(rows, init, acts):
+        return np.stack([o.simulate(np.asarray(r, np.float64), i0, a)[0] for r, i0, a in zip(rows, init, acts)])
+    segs = synthetic_segments(simulate, truth, sim, n_seg=n_seg, horizon=H, seed=2)
+    seg_dev = eng.upload_segments(segs)
+    problem = SysIdProblem(eng, seg_dev, H, max_cand=4096)
     cem = DeviceCEM(prior.mean, prior.std, prior.lo, prior.hi, prior.min_std, CEMConfig(4096, 0.1, 12, seed=1))
     th_cem = cem.solve(problem).cpu().numpy().astype(np.float64)
-    o = spio.Oracle(load_model(), sim)
-    th64, c64, hist = spio.gauss_newton_fp64(o, th_cem, _rows(IDENTIFIED), segs, eng.weights,
-                                             np.full(n_seg, 1.0 / n_seg), prior.lo, prior.hi, iterations=3)
-    rel = np.abs(th_cem[:22] - th64[:22]) / np.abs(th64[:22])
-    print("FP64 LM cost history", hist)
-    print({PARAM_NAMES[k]: round(float(rel[k]), 4) for k in range(22)})
-    best = [PARAM_NAMES.index(n) for n in ("mass", "kp_scale_FL", "kp_scale_FR", "kp_scale_RL", "kp_scale_RR")]
-    assert rel[best].max() < 1e-2, {PARAM_NAMES[k]: rel[k] for k in best}
+    res = GaussNewtonRefiner(eng, seg_dev, H, lo=prior.lo, hi=prior.hi).refine(th_cem, iterations=10)
+    rows = _rows(IDENTIFIED)
+    w = np.full(n_seg, 1.0 / n_seg)
+    th64, c64, hist = spio.gauss_newton_fp64(o, truth, rows, segs, eng.weights, w, prior.lo, prior.hi,
+                                             iterations=30, tol=1e-14)
+    # stationarity and standard errors at the FP64 minimiser
+    grp = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
+    y, J, _ = o.traj_sens_batch(th64, rows, segs.init, segs.actions, n_rows=37)
+    sw = np.sqrt(np.asarray(eng.weights)[grp][None, :] * w[:, None])
+    r = (sw[:, None, :] * (y - segs.targets.astype(np.float64))).ravel()
+    Jw = (sw[:, None, :, None] * J).reshape(-1, len(rows))
+    g = 2.0 * Jw.T @ r
+    rel_grad = np.abs(g) * np.maximum(np.abs(th64[rows]), 1e-3) / (r @ r)
+    se_rel = np.sqrt(np.diag((r @ r) / (r.size - len(rows)) * np.linalg.inv(Jw.T @ Jw))) / np.abs(th64[rows])
+    err = np.abs(res.theta[rows] - th64[rows]) / np.abs(th64[rows])
+    c_dev = o.rollout_cost(res.theta, segs.init, segs.actions, segs.targets, 0, np.asarray(eng.weights)).mean()
+    print("FP64 LM cost history", hist, "max relative gradient", rel_grad.max())
+    print("device pipeline cost (FP64 oracle)", c_dev, "FP64 minimum", c64)
+    for k, i in enumerate(rows):
+        print(f"{PARAM_NAMES[i]:14s} se {se_rel[k]:.2e} error {err[k]:.2e}")
+    assert len(hist) > 2 and rel_grad.max() <= 1e-3
+    assert c_dev <= c64 * (1 + 1e-4)
+    ident = [k for k in range(len(rows)) if se_rel[k] < 1e-2]
+    assert len(ident) >= 5                       # mass, the Kp scales, ... (16 short segments)
+    bad = {PARAM_NAMES[rows[k]]: float(err[k]) for k in ident if err[k] > 1e-2}
+    assert not bad, bad
+    # the two land on the same stationary point: every parameter, the poorly determined ones
+    # included, within 2 % (measured: 1.9e-3 at most, mass 3e-5)
+    assert err.max() <= 2e-2, {PARAM_NAMES[rows[k]]: float(err[k]) for k in range(len(rows)) if err[k] > 2e-2}

@@ -87,7 +87,10 @@ def test_host_adjoint_matches_differences_and_oracle(libs, model, sim, theta0, m
         assert np.abs(g - fd).max() < 1e-4 * scale, (s, np.abs(g - fd).max() / scale)
         cr, gr = o.cost_grad(theta, np.arange(25), segs.init[s:s + 1], segs.actions[s:s + 1],
                              segs.targets[s:s + 1], mode, w)
-        assert abs(cost.value - cr[0]) < 1e-6 * max(cr[0], 1e-3)
+        # the specialised code keeps the model constants in float (spi_go2_model.h): a ~1e-7
+        # relative model difference from the oracle's double tree, which the contacts of a
+        # rollout amplify to ~1e-6 of the cost (measured 7e-8 .. 1.6e-6 over these six segments)
+        assert abs(cost.value - cr[0]) < 5e-6 * max(cr[0], 1e-3)
         assert np.abs(g - gr[0]).max() < 1e-3 * scale
         # the float build (the device arithmetic) against the FP64 oracle
         assert np.abs(gf - gr[0]).max() < 2e-2 * scale, np.abs(gf - gr[0]).max() / scale

@@ -54,7 +54,9 @@ def test_host_forward_dynamics_matches_oracle(host_lib, model, sim, theta0):
         a0, qdd, fo, _ = o.forward(states[i, :13], states[i, 13:25], states[i, 25:37], tau[i], rows[i])
         ref = np.r_[a0, qdd]
         worst = max(worst, np.abs(acc - ref).max() / max(np.abs(ref).max(), 1.0))
-        assert np.allclose(ff.reshape(4, 3), fo, rtol=1e-4, atol=1e-3)
+        # the penetration is a difference of ~0.3 m heights: its FP32 rounding (~3e-8 m) times the
+        # contact stiffness (k = 5e4 N/m, up to 2k where the damping saturates) is ~3e-3 N
+        assert np.allclose(ff.reshape(4, 3), fo, rtol=1e-4, atol=1e-2)
     assert worst <= 1e-3, worst
 
 

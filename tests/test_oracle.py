@@ -280,6 +280,37 @@ def test_contact_force_known_answer(model, sim, theta0):
     assert ff[low, 2] > 0
 
 
+@pytest.mark.parametrize("vz", [-0.05, -1.0, 0.3, 3.0])
+def test_contact_force_bounded_damping_known_answer(model, sim, theta0, vz):
+    """Translating robot (base velocity (0, 0, vz), joints at rest): every penetrating foot gets
+    Fn = k d (1 - tanh(c vz / (k d))): about k d - c vz for a slow approach or separation, never
+    more than 2 k d however fast the approach, never a pull.  As d -> 0 the force goes to 0
+    whatever vz, the continuity of the law at touchdown (the unsaturated law
+    max(0, k d - c vz) would give c |vz| there)."""
+    o = spio.Oracle(model, sim)
+    s = np.zeros(37)
+    s[6] = 1.0
+    s[13:25] = sim.q0
+    s[9] = vz
+    _, feet = o.fk(s[:13], s[13:25])
+    k, c = sim.k_contact, sim.c_contact
+    for d in (0.004, 1e-4, 1e-7):
+        s[2] = -feet[:, 2].min() + 0.022 - d
+        _, _, ff, _ = o.forward(s[:13], s[13:25], np.zeros(12), np.zeros(12), theta0)
+        pen = 0.022 - (feet[:, 2] + s[2])
+        for f in range(4):
+            if pen[f] <= 0:
+                assert np.allclose(ff[f], 0.0)
+                continue
+            fn = k * pen[f] * (1.0 - np.tanh(c * vz / (k * pen[f])))
+            assert abs(ff[f, 2] - fn) <= 1e-9 * max(1.0, fn), (d, vz, ff[f, 2], fn)
+            assert 0.0 <= ff[f, 2] <= 2 * k * pen[f] + 1e-12
+            if d == 0.004 and vz == -0.05:          # |c vz| << k d: the plain spring-damper
+                assert abs(ff[f, 2] - (k * pen[f] - c * vz)) <= 0.02 * ff[f, 2]
+        if d == 1e-7:
+            assert np.abs(ff[:, 2]).max() <= 2 * k * 1e-7 + 1e-12
+
+
 @pytest.mark.parametrize("mode,motor", [(0, 0), (1, 3), (1, 1), (1, 2)])
 def test_control_torque_known_answers(model, sim, theta0, mode, motor):
     """Closed forms of legged_robot_base.py:542-560, go2_omni.py:436-462 and the act2tau models."""
