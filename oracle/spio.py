@@ -396,9 +396,15 @@ def cem_sample(mean, std, lo, hi, n, seed, iteration):
 
 
 def gauss_newton_fp64(o: "Oracle", theta0, rows, segs, weights, seg_w, lo=None, hi=None, iterations=8, lam0=1e-3,
-                      tol=1e-10):
+                      tol=1e-10, line_search=0, grad_tol=0.0):
     """FP64 Levenberg-Marquardt on the tracking residuals with the oracle's exact derivatives;
-    the same damping / acceptance rule as the GPU refiner (spi_amd.agents.refine)."""
+    the same damping / acceptance rule as the GPU refiner (spi_amd.agents.refine): the damped
+    Gauss-Newton step is evaluated at its full length and at 1/2, 1/4, ... (line_search
+    halvings); the lowest cost below the current one is taken, and the damping is divided by 3
+    if the full step won; if none is lower the damping is multiplied by 4.  The Jacobian is
+    recomputed only after a step was taken.  Stops after `iterations` iterations, when a step
+    gains less than tol * cost, or when the largest relative gradient
+    |g_i| max(|theta_i|, 1e-3) / cost falls to grad_tol."""
     rows = np.asarray(rows, dtype=np.int32)
     w = np.asarray(weights, np.float64)
     grp = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
@@ -419,23 +425,34 @@ def gauss_newton_fp64(o: "Oracle", theta0, rows, segs, weights, seg_w, lo=None, 
     cost = float(r @ r)
     lam = lam0
     history = [cost]
+    A = b = None
     for _ in range(iterations):
-        r, J = resid(theta, True)
-        A, b = J.T @ J, J.T @ r
+        if A is None:                                   # new point: Jacobian and normal equations
+            r, J = resid(theta, True)
+            A, b = J.T @ J, J.T @ r
+            rel_grad = np.abs(2.0 * b) * np.maximum(np.abs(theta[rows]), 1e-3) / max(cost, 1e-300)
+            if rel_grad.max() <= grad_tol:
+                break
         eps = 1e-12 * np.trace(A) / len(rows)
         Ad = A + lam * np.diag(np.diag(A)) + eps * np.eye(len(rows))
         delta = -np.linalg.solve(Ad, b)
-        cand = theta.copy()
-        cand[rows] += delta
-        if lo is not None:
-            cand = np.clip(cand, lo, hi)
-        rn, _ = resid(cand, False)
-        c_new = float(rn @ rn)
-        if np.isfinite(c_new) and c_new < cost:
+        best, c_new, k_best = None, np.inf, -1
+        for k in range(line_search + 1):
+            cand = theta.copy()
+            cand[rows] += delta * 0.5 ** k
+            if lo is not None:
+                cand = np.clip(cand, lo, hi)
+            rn, _ = resid(cand, False)
+            c_k = float(rn @ rn)
+            if np.isfinite(c_k) and c_k < c_new:
+                best, c_new, k_best = cand, c_k, k
+        if best is not None and c_new < cost:
             gain = cost - c_new
-            theta, cost = cand, c_new
-            lam = max(lam / 3.0, 1e-9)
+            theta, cost = best, c_new
+            if k_best == 0:
+                lam = max(lam / 3.0, 1e-9)
             history.append(cost)
+            A = b = None
             if gain <= tol * max(cost, 1e-30):
                 break
         else:

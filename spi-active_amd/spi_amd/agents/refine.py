@@ -67,11 +67,17 @@ class GaussNewtonRefiner:
     def _theta_dev(self, theta: np.ndarray) -> torch.Tensor:
         return torch.from_numpy(np.asarray(theta, np.float32).reshape(P_COUNT, 1).copy()).to(self.engine.device)
 
+    def objectives(self, thetas: np.ndarray) -> np.ndarray:
+        """Tracking costs sum_s w_s cost_s of k parameter vectors [k, 25], all in one fused launch
+        (k candidates x all segments)."""
+        th = np.atleast_2d(np.asarray(thetas, np.float32))
+        t = torch.from_numpy(np.ascontiguousarray(th.T)).to(self.engine.device)
+        c = self.engine.segment_costs(t, th.shape[0], self.segments, self.H, seg_gains=self.seg_gains)
+        return (self.seg_w[:, None] * c).sum(0).double().cpu().numpy()
+
     def objective(self, theta: np.ndarray) -> float:
         """Tracking cost sum_s w_s cost_s of one parameter vector (fused primal kernel)."""
-        t = self._theta_dev(theta)
-        c = self.engine.segment_costs(t, 1, self.segments, self.H, seg_gains=self.seg_gains)
-        return float((self.seg_w[:, None] * c).sum().item())
+        return float(self.objectives(theta)[0])
 
     def step(self, theta: np.ndarray, lam: float):
         """One damped Gauss-Newton step: (delta [P], cost at theta, predicted cost after it)."""
@@ -91,23 +97,34 @@ class GaussNewtonRefiner:
         _lib.check(rc, "spi_gn_step")
         return self.delta[0].cpu().numpy(), float(self.cost.item()), float(self.pred.item())
 
-    def refine(self, theta0: np.ndarray, iterations: int = 10, lam0: float = 1e-3, tol: float = 1e-10) -> RefineResult:
+    def refine(self, theta0: np.ndarray, iterations: int = 10, lam0: float = 1e-3, tol: float = 1e-10,
+               line_search: int = 4) -> RefineResult:
+        """Damped Gauss-Newton from theta0.  Each iteration solves for the step at the current
+        damping and evaluates it together with line_search shortened copies (1/2, 1/4, ... of
+        its length) in one launch; the lowest cost below the current one is taken (the damping
+        then drops by 3 if the full step won), otherwise the damping grows by 4.  The contacts
+        make the cost piecewise smooth, and near a crease of it the full step often overshoots
+        where a shorter one still descends."""
         theta = np.asarray(theta0, np.float64).copy()
         cost = self.objective(theta)
         res = RefineResult(theta.copy(), cost, [cost])
         lam = lam0
+        fracs = 0.5 ** np.arange(line_search + 1)
         for it in range(iterations):
             delta, c_here, pred = self.step(theta, lam)
-            cand = theta.copy()
-            cand[self.rows] += delta
+            cands = np.repeat(theta[None], len(fracs), axis=0)
+            cands[:, self.rows] += fracs[:, None] * delta[None, :]
             if self.lo is not None:
-                cand = np.clip(cand, self.lo, self.hi)
-            c_new = self.objective(cand)
+                cands = np.clip(cands, self.lo, self.hi)
+            c_new = self.objectives(cands)
+            c_new = np.where(np.isfinite(c_new), c_new, np.inf)
+            k = int(np.argmin(c_new))
             res.iterations = it + 1
-            if np.isfinite(c_new) and c_new < cost:
-                gain = cost - c_new
-                theta, cost = cand, c_new
-                lam = max(lam / 3.0, 1e-9)
+            if c_new[k] < cost:
+                gain = cost - c_new[k]
+                theta, cost = cands[k].copy(), float(c_new[k])
+                if k == 0:
+                    lam = max(lam / 3.0, 1e-9)
                 res.accepted += 1
                 res.history.append(cost)
                 if gain <= tol * max(cost, 1e-30):

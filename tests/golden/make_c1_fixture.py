@@ -9,19 +9,26 @@ Data: the seeded ground truth of the workload (identification_prior(), ground_tr
 sigma_v 0.05 rad/s) on the joint channels (spi_amd.data.synthetic_segments, seed 2), every
 trajectory simulated by the FP64 oracle.
 
-Minimiser: FP64 Levenberg-Marquardt (spio.gauss_newton_fp64, exact forward-mode Jacobians of
-the oracle's discrete rollout) of the tracking cost mean_s sum_t sum_i w_i (x_i - target_i)^2
-with the workload's weights (10, 5, 1, 1, 1, 0.05 for pos, quat, linvel, angvel, q, qd), over the
-22 identified rows inside the prior's bounds, started once from the truth and once from the
-prior mean; the lower of the two is the reference minimiser (the role of the reference's argmin
-in scripts/mass_opt.py:136-169 and scripts/mass_landscape.py:161-171), and the gradient there
-is recorded to show it is a stationary point.
+Minimiser of the tracking cost mean_s sum_t sum_i w_i (x_i - target_i)^2 with the workload's
+weights (10, 5, 1, 1, 1, 0.05 for pos, quat, linvel, angvel, q, qd) over the 22 identified rows
+inside the prior's bounds, found entirely in FP64 on the CPU and independently of the device:
+  * FP64 Levenberg-Marquardt (spio.gauss_newton_fp64: exact forward-mode Jacobians of the
+    oracle's discrete rollout, the device refiner's line search over halved steps) started from
+    the truth;
+  * the same method the device runs, in FP64: a cross-entropy search (1024 samples per
+    iteration from the oracle's Philox restatement of the device sampler, 10 % elites, 12
+    iterations, the prior as the initial distribution), then the FP64 Levenberg-Marquardt from
+    its mean.
+The lower of the two end points is the reference minimiser (the role of the reference's argmin
+in scripts/mass_opt.py:136-169 and scripts/mass_landscape.py:161-171).  The contacts make the
+objective piecewise smooth: a local search can end where no step lowers the cost although the
+gradient of the piece it is on is not zero; the gradient there is recorded.
 
 Per-parameter asymptotic standard error of the minimiser (Gaussian residuals: the Cramer-Rao
 bound of the nonlinear least-squares estimate): cov = s^2 (J^T J)^-1, s^2 = r.r / (N - P),
 with r and J the weighted residuals and their Jacobian at the minimiser.
 
-Run from the repository root: python tests/golden/make_c1_fixture.py  (about 5 minutes on 8 cores)
+Run from the repository root: python tests/golden/make_c1_fixture.py  (about an hour on 8 cores)
 """
 import os
 import sys
@@ -44,6 +51,7 @@ TRUTH_SEED, DATA_SEED = 1, 2
 WEIGHTS = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
 GROUP = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
 OUT = os.path.join(HERE, "c1_fp64_minimiser.npz")
+CEM_SAMPLES, CEM_ITERS, CEM_SEED = 1024, 12, 7
 
 
 def main():
@@ -66,15 +74,36 @@ def main():
     print(f"data: {time.time() - t0:.1f} s", flush=True)
     rows = np.arange(22)
     seg_w = np.full(N_SEG, 1.0 / N_SEG)
+
+    def lm(start):
+        return spio.gauss_newton_fp64(o, start, rows, segs, WEIGHTS, seg_w, prior.lo, prior.hi, iterations=60,
+                                      lam0=1e-3, tol=1e-14, line_search=4, grad_tol=1e-6)
+
     fits = {}
-    for name, start in (("truth", truth), ("prior_mean", prior.mean)):
-        t = time.time()
-        th, cost, hist = spio.gauss_newton_fp64(o, start, rows, segs, WEIGHTS, seg_w, prior.lo, prior.hi,
-                                                iterations=60, lam0=1e-3, tol=1e-14)
-        fits[name] = (th, cost, np.asarray(hist))
-        print(f"LM from {name}: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
-              f"({time.time() - t:.0f} s)", flush=True)
-    best = min(fits, key=lambda k: fits[k][1])          # the lower of the two local minima
+    t = time.time()
+    th, cost, hist = lm(truth)
+    fits["truth"] = (th, cost, np.asarray(hist))
+    print(f"LM from truth: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
+          f"({time.time() - t:.0f} s)", flush=True)
+    # FP64 cross-entropy search, the device CEM's update rule (elite mean, population std with
+    # the prior's floor) on samples from the oracle's restatement of the device sampler
+    t = time.time()
+    mean, std = prior.mean.copy(), prior.std.copy()
+    n_elite = int(np.ceil(0.1 * CEM_SAMPLES - 1e-9))
+    for it in range(CEM_ITERS):
+        theta = spio.cem_sample(mean, std, prior.lo, prior.hi, CEM_SAMPLES, CEM_SEED, it)      # [25, n]
+        c = o.rollout_cost_par(theta.T, segs.init, segs.actions, segs.targets, 0, WEIGHTS).mean(axis=1)
+        c = np.where(np.isfinite(c), c, np.inf)
+        el = theta[:, np.lexsort((np.arange(CEM_SAMPLES), c))[:n_elite]]
+        mean, std = el.mean(axis=1), np.maximum(el.std(axis=1), prior.min_std)
+        print(f"FP64 CEM iteration {it}: best {c.min():.6g}, elite mean cost {np.sort(c)[:n_elite].mean():.6g}",
+              flush=True)
+    cem_mean = mean.copy()
+    th, cost, hist = lm(cem_mean)
+    fits["fp64_cem"] = (th, cost, np.asarray(hist))
+    print(f"FP64 CEM + LM: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
+          f"({time.time() - t:.0f} s)", flush=True)
+    best = min(fits, key=lambda k: fits[k][1])          # the lower of the two end points
     th = fits[best][0]
     print(f"reference minimiser: LM from {best}", flush=True)
     y, J, rc = o.traj_sens_batch(th, rows, segs.init, segs.actions, n_rows=37)
@@ -90,17 +119,17 @@ def main():
     g = 2.0 * (Jw.T @ r)                                  # gradient of the cost r.r
     # first-order optimality: relative cost change per relative parameter change, at most
     rel_sens = np.abs(g) * np.maximum(np.abs(th[rows]), 1e-3) / (r @ r)
-    other = "prior_mean" if best == "truth" else "truth"
+    other = "fp64_cem" if best == "truth" else "truth"
     print(f"param            truth          fp64 minimiser   from {other:10s} se_rel")
     for i in rows:
         print(f"{PARAM_NAMES[i]:14s} {truth[i]: .8e} {th[i]: .8e} {fits[other][0][i]: .8e} {se_rel[i]:.3e}")
     print(f"gradient norm {np.linalg.norm(g):.3e}; max relative sensitivity {rel_sens.max():.3e}; rc {rc}")
     np.savez_compressed(
         OUT, segments=segs.packed(), truth=truth, theta_fp64=th, start=np.array(best),
-        theta_fp64_truth_start=fits["truth"][0], theta_fp64_prior_start=fits["prior_mean"][0],
+        theta_fp64_truth_start=fits["truth"][0], theta_fp64_cem_start=fits["fp64_cem"][0], fp64_cem_mean=cem_mean,
         cost_fp64=fits[best][1], cost_fp64_truth_start=fits["truth"][1],
-        cost_fp64_prior_start=fits["prior_mean"][1], history_truth_start=fits["truth"][2],
-        history_prior_start=fits["prior_mean"][2], cost_truth=fits["truth"][2][0], se=se, se_rel=se_rel,
+        cost_fp64_cem_start=fits["fp64_cem"][1], history_truth_start=fits["truth"][2],
+        history_cem_start=fits["fp64_cem"][2], cost_truth=fits["truth"][2][0], se=se, se_rel=se_rel,
         grad_at_minimiser=g, rows=rows, weights=WEIGHTS, n_seg=N_SEG, horizon=HORIZON, truth_seed=TRUTH_SEED,
         data_seed=DATA_SEED)
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {time.time() - t0:.0f} s)")

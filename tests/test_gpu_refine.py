@@ -76,7 +76,7 @@ def test_refine_matches_fp64_oracle_minimiser(gpu, sim, theta0):
     gpu_res = ref.refine(start, iterations=6)
     o = spio.Oracle(load_model(), sim)
     th64, c64, hist = spio.gauss_newton_fp64(o, start, _rows(IDENTIFIED), segs, eng.weights,
-                                             np.full(6, 1 / 6), prior.lo, prior.hi, iterations=6)
+                                             np.full(6, 1 / 6), prior.lo, prior.hi, iterations=6, line_search=4)
     print("gpu cost", gpu_res.history, "fp64 cost", hist)
     assert abs(gpu_res.cost - c64) < 2e-3 * c64
     big = np.abs(th64[:22]) > 1e-2
@@ -162,7 +162,7 @@ def test_cem_and_refinement_find_fp64_oracle_minimiser(gpu, model, sim):
     rows = _rows(IDENTIFIED)
     w = np.full(n_seg, 1.0 / n_seg)
     th64, c64, hist = spio.gauss_newton_fp64(o, truth, rows, segs, eng.weights, w, prior.lo, prior.hi,
-                                             iterations=30, tol=1e-14)
+                                             iterations=30, tol=1e-14, line_search=4)
     # stationarity and standard errors at the FP64 minimiser
     grp = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
     y, J, _ = o.traj_sens_batch(th64, rows, segs.init, segs.actions, n_rows=37)
