@@ -483,6 +483,16 @@ long long spi_mlp_param_floats(int n_layers, const int* dims);
 long long spi_mlp_work_floats(int M, int n_layers, const int* dims);
 int spi_mlp_forward(const float* x_dev, int M, int n_layers, const int* dims, const float* params_dev, float* work_dev,
                     long long work_floats, float* y_dev, hipStream_t stream);
+/* spi_mlp_forward without a caller workspace and without any allocation: no counterpart in
+ * the reference; for callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  The hidden activations live in a library-owned device
+ * array of SPI_MLP_SCRATCH_FLOATS floats per GPU (held until the process ends), so
+ * 2 * M * (widest hidden layer) must not exceed it (SPI_ERR_TOO_LARGE otherwise; 4096 rows of
+ * the 512-wide actor).  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other host threads) race.  Same results as spi_mlp_forward, bit for bit. */
+#define SPI_MLP_SCRATCH_FLOATS (1 << 22)
+int spi_mlp_forward_noalloc(const float* x_dev, int M, int n_layers, const int* dims, const float* params_dev,
+                            float* y_dev, hipStream_t stream);
 /* The same network with its input row m gathered from a ring (see spi_policy_terms):
  * x[m][k] = clip(ring_dev[((head - kmap[k][0]) mod n_slots) * n_terms + kmap[k][1]][m], +-clip),
  * kmap_dev [dims[0]][2] int (8-byte aligned); needs dims[0] % 4 == 0. */

@@ -11,8 +11,15 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 
 #include "spi_types.h"
+
+// Workspace of spi_mlp_forward_noalloc: the ping-pong hidden activations live in this
+// file-scope device array (one copy per GPU, held until the process ends) instead of a buffer
+// the caller allocates.  In host code the name stands for the symbol, not for a device
+// address: the address of the current device's copy comes from hipGetSymbolAddress.
+static __device__ __align__(256) float scratch[SPI_MLP_SCRATCH_FLOATS];
 
 namespace {
 
@@ -479,6 +486,40 @@ int spi_mlp_forward(const float* x_dev, int M, int n_layers, const int* dims, co
   if (!x_dev) return mfail(SPI_ERR_INVALID_ARG, "spi_mlp_forward: null input");
   const RingSrc none{nullptr, 0, 0, 0, 0, nullptr, 0.0f};
   return mlp_layers(x_dev, none, M, n_layers, dims, params_dev, work_dev, work_floats, y_dev, stream);
+}
+
+int spi_mlp_forward_noalloc(const float* x_dev, int M, int n_layers, const int* dims, const float* params_dev,
+                            float* y_dev, hipStream_t stream) {
+  // device address of this GPU's scratch array, looked up once per device with it current
+  static std::mutex mu;
+  static float* addr[64] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= 64)
+    return mfail(SPI_ERR_HIP, "spi_mlp_forward_noalloc: no current device");
+  float* work = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+      void* ptr = nullptr;
+      e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr) {
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "spi_mlp_forward_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+        return mfail(SPI_ERR_HIP, buf);
+      }
+      addr[dev] = static_cast<float*>(ptr);
+    }
+    work = addr[dev];
+  }
+  if (!x_dev) return mfail(SPI_ERR_INVALID_ARG, "spi_mlp_forward_noalloc: null input");
+  if (M < 0 || param_floats(n_layers, dims) < 0)
+    return mfail(SPI_ERR_INVALID_ARG, "spi_mlp_forward_noalloc: bad arguments");
+  if (2LL * M * widest_hidden(n_layers, dims) > SPI_MLP_SCRATCH_FLOATS)
+    return mfail(SPI_ERR_TOO_LARGE, "spi_mlp_forward_noalloc: 2 * M * (widest hidden layer) exceeds "
+                                    "SPI_MLP_SCRATCH_FLOATS; use spi_mlp_forward with a caller workspace");
+  const RingSrc none{nullptr, 0, 0, 0, 0, nullptr, 0.0f};
+  return mlp_layers(x_dev, none, M, n_layers, dims, params_dev, work, SPI_MLP_SCRATCH_FLOATS, y_dev, stream);
 }
 
 int spi_mlp_forward_ring(const float* ring_dev, int ld, int n_slots, int n_terms, int head, const int* kmap_dev,

@@ -20,6 +20,7 @@ SPI_P_COUNT = 25
 SPI_STATE_DIM = 37
 SPI_MAX_CEM_SAMPLES = 8192
 SPI_MAX_TANGENTS = 32
+SPI_MLP_SCRATCH_FLOATS = 1 << 22
 
 
 class SpiSimArgs(C.Structure):
@@ -153,6 +154,7 @@ def lib() -> C.CDLL:
                                     vp, vp, vp, vp, vp],
         "spi_policy_obs_kmap": [C.POINTER(ip)],
         "spi_reset_states": [C.POINTER(SpiResetSpec), ip, vp, C.c_ulonglong, vp, vp],
+        "spi_mlp_forward_noalloc": [vp, ip, ip, C.POINTER(ip), vp, vp, vp],
     }
     L.spi_adjoint_work_floats.argtypes = [ip, ip, ip, ip]
     L.spi_adjoint_legs_work_floats.argtypes = [ip, ip, ip, ip, ip]
