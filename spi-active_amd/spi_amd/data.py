@@ -212,6 +212,47 @@ def segment_gains(kp: np.ndarray, kd: np.ndarray, batch_size: Optional[int] = No
     return np.ascontiguousarray(g, np.float32)
 
 
+# ---------------------------------------------------------------- the reference's data recipes
+# scripts/data/sine.py:15-21, jump.py:15-21, stand.py:13-14: 5 s at the control rate, thigh
+# joints (1, 4, 7, 10) driven with raw action A sin(2 pi f t), calf joints (2, 5, 8, 11) with
+# its negative, hips 0; the sine and jump runs start with the base raised to 0.32 m.
+RECIPES = {"sine": (0.8, 1.0, 0.32), "jump": (1.2, 1.5, 0.32), "stand": (0.0, 0.0, None)}
+THIGH_IDX = (1, 4, 7, 10)
+CALF_IDX = (2, 5, 8, 11)
+
+
+def recipe_actions(name: str, dt_ctrl: float, duration: float = 5.0) -> np.ndarray:
+    """[steps, 12] raw actions of a reference recipe: steps = round(duration / dt_ctrl), action at
+    step k uses t = k dt_ctrl (sine.py:35-46)."""
+    if name not in RECIPES:
+        raise ValueError(f"unknown recipe {name!r}; one of {sorted(RECIPES)}")
+    amp, freq, _ = RECIPES[name]
+    steps = int(round(duration / dt_ctrl))
+    phase = amp * np.sin(2.0 * np.pi * freq * np.arange(steps) * dt_ctrl)
+    a = np.zeros((steps, 12), np.float32)
+    a[:, list(THIGH_IDX)] = phase[:, None]
+    a[:, list(CALF_IDX)] = -phase[:, None]
+    return a
+
+
+def record_recipe(simulate: SimulateFn, theta: np.ndarray, sim: SimParams, name: str, init: np.ndarray,
+                  duration: float = 5.0) -> Tuple[np.ndarray, np.ndarray]:
+    """A recipe run recorded the way the reference's collectors do (sine.py:37-70, common.py):
+    ``init`` is the state after the env's reset (scripts/data/common.py create_env: reset_all, then
+    for sine and jump the base height set to 0.32 m), every step applies the recipe's action and
+    frame k holds the state AFTER step k beside action k.  Returns (states [T, 37], actions
+    [T, 12]) for ``save_npz_trajectory``.  (scripts/eval.py:101-171 then replays window t from the
+    state after step t with actions t .. t + H - 1: the reference's own pairing, kept as is.)"""
+    acts = recipe_actions(name, sim.dt * sim.decimation, duration)
+    s0 = np.asarray(init, np.float32).copy()
+    height = RECIPES[name][2]
+    if height is not None:
+        s0[2] = height
+    theta = np.asarray(theta, np.float64).reshape(1, -1)
+    states = simulate(theta, s0[None], acts[None])[0]
+    return states.astype(np.float32), acts
+
+
 def save_npz_trajectory(path: str, states: np.ndarray, actions: np.ndarray, dt: float, kp: float, kd: float) -> None:
     """Write one trajectory in the reference's npz schema (scripts/data/common.py:83-101)."""
     T = states.shape[0]

@@ -354,3 +354,43 @@ def test_empty_inputs_return_ok_and_oversized_are_refused():
     # the elite sort runs in LDS: at most SPI_MAX_CEM_SAMPLES samples per update
     assert L.spi_cem_update(f, f, 8193, 22, 8193, 820, 0.0, f, f, f, f, f, None, None) == TOO_LARGE
     assert b"spi_cem_update" in L.spi_last_error()
+
+
+def test_reference_recipes_known_answers(tmp_path, model, sim, theta0):
+    """The reference's data recipes (scripts/data/sine.py, jump.py, stand.py) with the oracle as
+    the simulator: 250 steps of 20 ms, thigh actions A sin(2 pi f t) with (A, f) = (0.8, 1),
+    (1.2, 1.5), calf actions their negative, hips zero; sine and jump start at base height
+    0.32 m; frame k is the state after step k (continuing the oracle's own rollout from frame
+    k - 1 with action k reproduces frame k); the npz written in the reference's schema loads into
+    250 - H windows."""
+    import spio
+    from spi_amd.data import (RECIPES, load_npz_dataset, record_recipe, recipe_actions, save_npz_trajectory,
+                              standing_state)
+    o = spio.Oracle(model, sim)
+
+    def simulate(rows, init, acts):
+        return np.stack([o.simulate(np.asarray(r, np.float64), i0, a)[0] for r, i0, a in zip(rows, init, acts)])
+
+    dt = sim.dt * sim.decimation
+    for name, (amp, freq, height) in RECIPES.items():
+        a = recipe_actions(name, dt)
+        assert a.shape == (250, 12)
+        t = np.arange(250) * dt
+        assert np.allclose(a[:, 1], amp * np.sin(2 * np.pi * freq * t), atol=1e-6)
+        assert np.array_equal(a[:, [4, 7, 10]], np.repeat(a[:, 1:2], 3, axis=1))
+        assert np.array_equal(a[:, [2, 5, 8, 11]], -np.repeat(a[:, 1:2], 4, axis=1))
+        assert np.all(a[:, [0, 3, 6, 9]] == 0.0)
+    init = standing_state(0.30)
+    states, acts = record_recipe(simulate, theta0, sim, "sine", init, duration=0.4)
+    assert states.shape == (20, 37) and acts.shape == (20, 12)
+    s0 = init.copy()
+    s0[2] = 0.32
+    first, _ = o.simulate(theta0, s0, acts[:1])
+    assert np.array_equal(first[0], states[0])
+    again, _ = o.simulate(theta0, states[9], acts[10:11])
+    assert np.abs(again[0] - states[10]).max() < 1e-5
+    f = str(tmp_path / "sine.npz")
+    save_npz_trajectory(f, states, acts, dt, 20.0, 0.5)
+    ds = load_npz_dataset([f], 5)
+    assert ds.n_seg == 15 and np.array_equal(ds.init[3], states[3]) and np.array_equal(ds.targets[3, -1], states[8])
+    assert np.array_equal(ds.actions[3, 0], acts[3])
