@@ -31,7 +31,7 @@ METRIC = "env-steps/s, sysID solves/s at ≤1% param error; p50/p99 solve latenc
 # Algorithmic FLOPs per control step of the fused rollout (PD + motor model + 4 x 4 ABA
 # sub-intervals + contact + integration + tracking cost), from the census program
 # spi-active_amd/csrc/spi_flop_census.cpp (counting scalar over the kernel's own code).
-FLOPS_PER_CONTROL_STEP = 84689.2
+FLOPS_PER_CONTROL_STEP = 84690.6
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 (vector) peak
 # HBM bytes per rollout_cost_kernel launch of this workload: rocprofv3 --pmc FETCH_SIZE
 # (2,338.8 KiB, x 2.000 from a 256 MiB copy's reading) + WRITE_SIZE (4,096.0 KiB, x 1.000).

@@ -670,6 +670,8 @@ SPI_HD bool below_limit(float x, float limit) {
   return x < limit;
 #endif
 }
+// other scalar types (host builds in double, the operation-counting type)
+template <class T> SPI_HD bool below_limit(const T& x, float limit) { return x < T(limit); }
 
 SPI_HD Real state_norm(const State& st) {
   Real s = fabsf(st.w.x) + fabsf(st.w.y) + fabsf(st.w.z) + fabsf(st.v.x) + fabsf(st.v.y) + fabsf(st.v.z);

@@ -14,8 +14,14 @@
 // device compiler and is not counted.  So the census counts what the algorithm needs per
 // rollout, not what a generic dense implementation would do.
 //
-// Build and run:  hipcc --offload-host-only -O2 -std=c++17 -I include -I spi-active_amd/csrc \
+// It then replays segment records cut from the same excitation through the reverse-mode cost
+// + gradient of one rollout (spi_adj.h rollout_grad on the same counting scalar: forward pass
+// with checkpoints, block recompute, physics-step and sub-interval recompute, sub-interval
+// adjoints) and prints its FLOPs per control step (H 20, checkpoints every 10 control steps).
+//
+// Build and run:  hipcc --offload-host-only -O0 -std=c++17 -I include -I spi-active_amd/csrc \
 //                 spi-active_amd/csrc/spi_flop_census.cpp -o /tmp/census && /tmp/census
+// (-O0: optimising the counting-scalar instantiation of the adjoint takes the compiler many minutes)
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -89,6 +95,17 @@ inline bool isfinite(CountF a) { return std::isfinite(a.v); }
 
 #define SPI_REAL_T census::CountF
 #include "spi_dyn.h"
+#include "spi_adj.h"
+
+#include <vector>
+
+namespace census {
+struct CountStore {              // the adjoint's checkpoint store (device: a strided HBM buffer)
+  std::vector<spi::State> s;
+  void put(int k, const spi::State& x) { s[k] = x; }
+  void get(int k, spi::State& x) { x = s[k]; }
+};
+}  // namespace census
 
 using census::CountF;
 
@@ -118,6 +135,11 @@ int main() {
   for (int g = 0; g < 3; ++g) lp.motor[g] = CountF::input(20.f);
 
   const int n_traj = 8, settle = 50, steps = 50;   // 1 s of excitation per trajectory
+  // segment records of the rollout kernels' layout cut from the excitation (runtime values):
+  // the adjoint census replays them (the 22-parameter gradient workload: H 20, checkpoints
+  // every 10 control steps)
+  const int H = 20, every = 10;
+  std::vector<CountF> recs;
   long long ctrl_flops = 0, contact_feet = 0, physics_steps = 0;
   float min_height = 1e9f;
   CountF tgt[37];
@@ -153,6 +175,18 @@ int main() {
       CountF c = step_cost(cost, st, tgt);
       (void)c;
       (void)state_norm(st);
+      if (t == settle + 10) {                  // a record starts here: init state, then H x (action, target)
+        CountF r[13], q[12], qd[12];
+        state_to_root(st, r, q, qd, 1);
+        for (int i = 0; i < 13; ++i) recs.push_back(CountF::input(r[i].v));
+        for (int j = 0; j < 12; ++j) recs.push_back(CountF::input(q[j].v));
+        for (int j = 0; j < 12; ++j) recs.push_back(CountF::input(qd[j].v));
+        for (int k = 0; k < H; ++k) {
+          const float tt = 0.02f * (t + 1 + k - settle);
+          for (int j = 0; j < 12; ++j) recs.push_back(CountF::input(A[j] * std::sin(6.2831853f * F[j] * tt + P[j]) / 0.25f));
+          for (int e = 0; e < 37; ++e) recs.push_back(CountF::input(0.01f * e));
+        }
+      }
       long long after = census::g_add + census::g_mul + census::g_div + census::g_trans;
       if (t >= settle) {
         ctrl_flops += after - before;
@@ -168,5 +202,24 @@ int main() {
               double(contact_feet) / physics_steps, census::g_add, census::g_mul, census::g_div, census::g_trans,
               n_traj * steps);
   std::printf("lowest base height during excitation %.4f m\n", min_height);
+
+  // reverse mode: cost + gradient with respect to all 25 parameter rows of one rollout
+  // (spi_adj.h rollout_grad: forward pass with checkpoints, block recompute, physics-step and
+  // sub-interval recompute, sub-interval adjoints), per control step
+  const int stride = SPI_STATE_DIM + H * (12 + SPI_STATE_DIM);
+  const int n_rec = int(recs.size()) / stride;
+  const CountF raw[4] = {lp.base.m, CountF::input(0.021112f), CountF::input(0.f), CountF::input(-0.005366f)};
+  census::g_add = census::g_mul = census::g_div = census::g_trans = 0;
+  for (int r = 0; r < n_rec; ++r) {
+    census::CountStore store;
+    store.s.resize((H + every - 1) / every + every);
+    State ps[8], ss[8];
+    ParBar pb;
+    CountF cst(0.f);
+    rollout_grad(a, cost, lp, raw, recs.data() + size_t(r) * stride, H, every, store, ps, ss, cst, pb);
+  }
+  const double adj = double(census::g_add + census::g_mul + census::g_div + census::g_trans) / (n_rec * H);
+  std::printf("{\"adjoint_flops_per_control_step\": %.1f, \"adjoint_over_primal\": %.2f, \"records\": %d, "
+              "\"horizon\": %d, \"checkpoint_every\": %d}\n", adj, adj / per_ctrl, n_rec, H, every);
   return 0;
 }
