@@ -24,11 +24,18 @@ in scripts/mass_opt.py:136-169 and scripts/mass_landscape.py:161-171).  The cont
 objective piecewise smooth: a local search can end where no step lowers the cost although the
 gradient of the piece it is on is not zero; the gradient there is recorded.
 
+The device pipeline's own run in FP64 (keys *_same_seed): the cross-entropy search with the
+device's configuration and seed (4096 samples, 410 elites, 12 iterations, seed 0: the same
+Philox draws as the device), its mean and std after every iteration, then the FP64
+Levenberg-Marquardt from its mean.  Against it the test compares the device's run of the same
+algorithm on the same inputs, FP32 against FP64.
+
 Per-parameter asymptotic standard error of the minimiser (Gaussian residuals: the Cramer-Rao
 bound of the nonlinear least-squares estimate): cov = s^2 (J^T J)^-1, s^2 = r.r / (N - P),
 with r and J the weighted residuals and their Jacobian at the minimiser.
 
-Run from the repository root: python tests/golden/make_c1_fixture.py  (about an hour on 8 cores)
+Run from the repository root: python tests/golden/make_c1_fixture.py  (about four hours on 8
+cores); --same-seed-only adds the same-seed run to an existing fixture (about three hours).
 """
 import os
 import sys
@@ -52,9 +59,42 @@ WEIGHTS = np.array([10.0, 5.0, 1.0, 1.0, 1.0, 0.05])
 GROUP = np.r_[np.zeros(3), np.ones(4), 2 * np.ones(3), 3 * np.ones(3), 4 * np.ones(12), 5 * np.ones(12)].astype(int)
 OUT = os.path.join(HERE, "c1_fp64_minimiser.npz")
 CEM_SAMPLES, CEM_ITERS, CEM_SEED = 1024, 12, 7
+DEVICE_SAMPLES, DEVICE_SEED = 4096, 0            # the device pipeline's configuration
 
 
-def main():
+def fp64_cem(o, segs, prior, n_samples, seed, label, n_threads=0):
+    """FP64 cross-entropy search: the device CEM's update rule (elites by (cost, index), elite
+    mean, population std with the prior's floor, no smoothing) on samples from the oracle's
+    restatement of the device sampler.  Returns the mean and std after every iteration."""
+    mean, std = prior.mean.copy(), prior.std.copy()
+    n_elite = int(np.ceil(0.1 * n_samples - 1e-9))
+    means, stds = [], []
+    for it in range(CEM_ITERS):
+        theta = spio.cem_sample(mean, std, prior.lo, prior.hi, n_samples, seed, it)      # [25, n]
+        c = o.rollout_cost_par(theta.T, segs.init, segs.actions, segs.targets, 0, WEIGHTS,
+                               n_threads=n_threads).mean(axis=1)
+        c = np.where(np.isfinite(c), c, np.inf)
+        el = theta[:, np.lexsort((np.arange(n_samples), c))[:n_elite]]
+        mean, std = el.mean(axis=1), np.maximum(el.std(axis=1), prior.min_std)
+        means.append(mean.copy())
+        stds.append(std.copy())
+        print(f"{label} iteration {it}: best {c.min():.6g}, elite mean cost {np.sort(c)[:n_elite].mean():.6g}",
+              flush=True)
+    return np.array(means), np.array(stds)
+
+
+def same_seed_run(o, segs, prior, lm):
+    t = time.time()
+    means, stds = fp64_cem(o, segs, prior, DEVICE_SAMPLES, DEVICE_SEED, "FP64 CEM (device seed)")
+    th, cost, hist = lm(means[-1])
+    print(f"FP64 CEM (device seed) + LM: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
+          f"({time.time() - t:.0f} s)", flush=True)
+    return {"cem_means_same_seed": means, "cem_stds_same_seed": stds, "theta_fp64_same_seed": th,
+            "cost_fp64_same_seed": cost, "history_same_seed": np.asarray(hist),
+            "same_seed_config": np.array([DEVICE_SAMPLES, CEM_ITERS, DEVICE_SEED])}
+
+
+def main(same_seed_only=False):
     t0 = time.time()
     model, sim = load_model(), SimParams()
     o = spio.Oracle(model, sim)
@@ -79,6 +119,16 @@ def main():
         return spio.gauss_newton_fp64(o, start, rows, segs, WEIGHTS, seg_w, prior.lo, prior.hi, iterations=60,
                                       lam0=1e-3, tol=1e-14, line_search=4, grad_tol=1e-6)
 
+    if same_seed_only:
+        with np.load(OUT, allow_pickle=False) as f:
+            old = {k: f[k] for k in f.files}
+        if not np.array_equal(old["segments"], segs.packed()):
+            raise RuntimeError(f"{OUT} holds other data than this script makes; regenerate it whole")
+        old.update(same_seed_run(o, segs, prior, lm))
+        np.savez_compressed(OUT, **old)
+        print(f"updated {OUT} ({os.path.getsize(OUT)} bytes, {time.time() - t0:.0f} s)")
+        return
+
     fits = {}
     t = time.time()
     th, cost, hist = lm(truth)
@@ -88,17 +138,7 @@ def main():
     # FP64 cross-entropy search, the device CEM's update rule (elite mean, population std with
     # the prior's floor) on samples from the oracle's restatement of the device sampler
     t = time.time()
-    mean, std = prior.mean.copy(), prior.std.copy()
-    n_elite = int(np.ceil(0.1 * CEM_SAMPLES - 1e-9))
-    for it in range(CEM_ITERS):
-        theta = spio.cem_sample(mean, std, prior.lo, prior.hi, CEM_SAMPLES, CEM_SEED, it)      # [25, n]
-        c = o.rollout_cost_par(theta.T, segs.init, segs.actions, segs.targets, 0, WEIGHTS).mean(axis=1)
-        c = np.where(np.isfinite(c), c, np.inf)
-        el = theta[:, np.lexsort((np.arange(CEM_SAMPLES), c))[:n_elite]]
-        mean, std = el.mean(axis=1), np.maximum(el.std(axis=1), prior.min_std)
-        print(f"FP64 CEM iteration {it}: best {c.min():.6g}, elite mean cost {np.sort(c)[:n_elite].mean():.6g}",
-              flush=True)
-    cem_mean = mean.copy()
+    cem_mean = fp64_cem(o, segs, prior, CEM_SAMPLES, CEM_SEED, "FP64 CEM")[0][-1]
     th, cost, hist = lm(cem_mean)
     fits["fp64_cem"] = (th, cost, np.asarray(hist))
     print(f"FP64 CEM + LM: {len(hist) - 1} accepted steps, cost {hist[0]:.6g} -> {cost:.9g} "
@@ -131,9 +171,9 @@ def main():
         cost_fp64_cem_start=fits["fp64_cem"][1], history_truth_start=fits["truth"][2],
         history_cem_start=fits["fp64_cem"][2], cost_truth=fits["truth"][2][0], se=se, se_rel=se_rel,
         grad_at_minimiser=g, rows=rows, weights=WEIGHTS, n_seg=N_SEG, horizon=HORIZON, truth_seed=TRUTH_SEED,
-        data_seed=DATA_SEED)
+        data_seed=DATA_SEED, **same_seed_run(o, segs, prior, lm))
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes, {time.time() - t0:.0f} s)")
 
 
 if __name__ == "__main__":
-    main()
+    main(same_seed_only="--same-seed-only" in sys.argv[1:])
