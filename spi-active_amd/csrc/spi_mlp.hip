@@ -2,10 +2,11 @@
 // ELU between them, spigym/agents/modules/modules.py:47-63; hidden sizes 512-256-128 from
 // config/algo/ppo.yaml) as FP32 MFMA (v_mfma_f32_32x32x2f32) tiles, one launch per layer with
 // bias and ELU in the epilogue.  linear_kernel: a workgroup of 4 wavefronts owns a 32-row x
-// 128-column output tile (many workgroups for small batches); linear_big_kernel: 128 x 128
-// (a 64 x 64 quadrant per wavefront) for batches of 8192 rows and more.  Rows of X and the
-// weight rows are staged through LDS in K chunks of 64 (even columns, then odd ones, 65 words per
-// row, so the operand reads of a wavefront hit 64 distinct banks).  Strict IEEE float32 (no fast-math).
+// 128-column output tile (many workgroups for small batches); rows of X and the weight rows are
+// staged through LDS in K chunks of 64 (even columns, then odd ones, 65 words per row, so the
+// operand reads of a wavefront hit 64 distinct banks).  linear_db_kernel: 128 x 128 (a 64 x 64
+// quadrant per wavefront, K chunks of 32 double-buffered in LDS) for batches of 8192 rows and
+// more.  Strict IEEE float32 (no fast-math).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -30,7 +31,7 @@ constexpr int kKc = 64;      // K chunk staged per pass
 constexpr int kPad = kKc + 1;
 constexpr int kMaxLayers = 8;
 constexpr int kMaxRingK = 1024;  // widest ring-gathered input
-constexpr int kBigM = 8192;      // batches from here on use linear_big_kernel (N >= 256)
+constexpr int kBigM = 8192;      // batches from here on use linear_db_kernel (N >= 256)
 
 // LDS slot of tile element (row, k): the even columns of a row first, then the odd ones, so
 // the MFMA operand reads of a wavefront (lane -> row l % 32, column kk + l / 32) cover 64
@@ -229,24 +230,33 @@ int split_for(int M, int K, int N) {
 
 // Large batches: a workgroup of 4 wavefronts owns a 128 x 128 output tile, each wavefront a
 // 64 x 64 quadrant (2 x 2 MFMA blocks, 4 accumulators), so every LDS operand read feeds two
-// MFMAs and every staged byte 2 MFLOP / 64 KB of work.  K chunks of 64 as in linear_kernel,
-// the next chunk prefetched into registers.  kSrc 1: X row-major (rows 16-byte aligned,
-// K % 4 == 0); 2: X gathered from a RingSrc (64 consecutive rows of one column per load).
-constexpr int kBigRows = 128, kBigCols = 128;
+// MFMAs.  K is staged in chunks of 32 through two LDS buffers (rows of 32 floats + 2 of padding,
+// 68 KB in all, so two workgroups share a CU): while the wavefronts multiply chunk c out of one
+// buffer, chunk c + 1 goes from registers into the other and chunk c + 2 is fetched into
+// registers, with one barrier per chunk.  In MFMA step s of a chunk the lanes of half h supply
+// column 16 h + s of both operands, read as float2 pairs (s, s + 1); in the last, ragged chunk
+// only the steps that reach a column below K run.  kSrc 0: X row-major (rows 16-byte aligned,
+// K % 4 == 0), float4 loads along the rows; 1: X gathered from a RingSrc whose lines are
+// 16-byte aligned with ld % 4 == 0, float4 loads of 4 rows of one column (lane l of a wavefront
+// covers rows 4 (l % 8) .. + 3 and column l / 8, so that the 64 lanes' LDS stores of one row
+// offset fall in 64 distinct banks); 2: any RingSrc, one row's four consecutive columns per
+// thread in scalar loads.  The accumulation order over K depends only on the tile layout, not
+// on the source, so the dense and the ring input give the same bits.
+constexpr int kDbRows = 128, kDbCols = 128, kDbK = 32, kDbStride = kDbK + 2, kDbTile = 128 * kDbStride;
 template <bool kElu, int kSrc>
-__global__ __launch_bounds__(256) void linear_big_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M,
-                                                         int K, const float* __restrict__ W,
-                                                         const float* __restrict__ b, int N, float* __restrict__ Y,
-                                                         int ldy) {
-  __shared__ float As[kBigRows * kPad];
-  __shared__ float Ws[kBigCols * kPad];
-  __shared__ int Kr[kSrc >= 2 ? kMaxRingK : 1];
-  const int m0 = blockIdx.x * kBigRows, n0 = blockIdx.y * kBigCols;
+__global__ __launch_bounds__(256) void linear_db_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M,
+                                                        int K, const float* __restrict__ W,
+                                                        const float* __restrict__ b, int N, float* __restrict__ Y,
+                                                        int ldy) {
+  __shared__ __attribute__((aligned(16))) float lds[2 * 2 * kDbTile];   // [buffer][A | W][row][34]
+  constexpr bool kRing = kSrc != 0;
+  __shared__ int Kr[kRing ? kMaxRingK : 1];
+  const int m0 = blockIdx.x * kDbRows, n0 = blockIdx.y * kDbCols;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int col = lane & 31, half = lane >> 5;
   const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;   // this wavefront's quadrant
   const bool live = n0 + wc < N;                            // uniform per wavefront
-  if (kSrc >= 2) {
+  if (kRing) {
     for (int k = tid; k < K; k += 256) {
       const int2 km = rs.kmap[k];
       const int back = min(max(km.x, 0), rs.n_slots - 1);
@@ -262,90 +272,102 @@ __global__ __launch_bounds__(256) void linear_big_kernel(const float* __restrict
     for (int q = 0; q < 2; ++q)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[p][q][r] = 0.0f;
-  // row-major float4 staging: 16 threads per 64-float row, 16 rows per pass
-  const int r0 = tid >> 4, c0 = (tid & 15) * 4;
-  constexpr int kPer = 8;                                  // float4 per thread per operand
-  // ring gather: thread -> row tid % 128, columns tid / 128 + 2 i
-  const int gm_r = m0 + (tid & 127), kk_r = tid >> 7;
-  constexpr int kGPer = kBigRows * kKc / 256;              // 32
-  // float4 ring gather: thread -> rows 4 (tid % 32) .. + 3, columns tid / 32 + 8 i
-  const int m4 = (tid & 31) * 4, kk4 = tid >> 5;
-  float ra[kSrc == 2 ? kGPer : 4 * kPer];
-  float rw[4 * kPer];
+  // W (and dense X) staging: thread -> rows (tid >> 3) + 32 i, columns 4 (tid & 7) .. + 3
+  const int sr = tid >> 3, sc = (tid & 7) * 4;
+  // float4 ring staging: thread -> rows 32 wave + 4 (lane % 8) .. + 3, columns lane / 8 + 8 i
+  const int fr = 32 * wave + 4 * (lane & 7), fk = lane >> 3;
+  // scalar ring staging: thread -> row tid & 127, column quads (tid >> 7) + 2 i
+  const int gr = tid & 127, gq = tid >> 7;
+  float4 ra[4], rw[4];
   auto fetch = [&](int k0) {
-    const int gk = k0 + c0;
-    if (kSrc == 2) {
 #pragma unroll
-      for (int i = 0; i < kGPer; ++i) {
-        const int k = k0 + kk_r + 2 * i;
-        float v = 0.0f;
-        if (gm_r < M && k < K) v = fminf(fmaxf(rs.ring[static_cast<size_t>(Kr[k]) * rs.ld + gm_r], -rs.clip), rs.clip);
-        ra[i] = v;
-      }
-    } else if (kSrc == 3) {
-#pragma unroll
-      for (int i = 0; i < kPer; ++i) {
-        const int k = k0 + kk4 + 8 * i;
+    for (int i = 0; i < 4; ++i) {
+      if (kSrc == 1) {
+        // rows up to M rounded to 4 exist in every line (ld >= that); rows >= M read as 0
+        const int gm = m0 + fr, k = k0 + fk + 8 * i;
         float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (m0 + m4 < M && k < K)
-          v = *reinterpret_cast<const float4*>(rs.ring + static_cast<size_t>(Kr[k]) * rs.ld + m0 + m4);
-        ra[4 * i] = fminf(fmaxf(v.x, -rs.clip), rs.clip);
-        ra[4 * i + 1] = fminf(fmaxf(v.y, -rs.clip), rs.clip);
-        ra[4 * i + 2] = fminf(fmaxf(v.z, -rs.clip), rs.clip);
-        ra[4 * i + 3] = fminf(fmaxf(v.w, -rs.clip), rs.clip);
-      }
-    } else {
+        if (gm < M && k < K) v = *reinterpret_cast<const float4*>(rs.ring + static_cast<size_t>(Kr[k]) * rs.ld + gm);
+        ra[i].x = fminf(fmaxf(v.x, -rs.clip), rs.clip);
+        ra[i].y = gm + 1 < M ? fminf(fmaxf(v.y, -rs.clip), rs.clip) : 0.0f;
+        ra[i].z = gm + 2 < M ? fminf(fmaxf(v.z, -rs.clip), rs.clip) : 0.0f;
+        ra[i].w = gm + 3 < M ? fminf(fmaxf(v.w, -rs.clip), rs.clip) : 0.0f;
+      } else if (kSrc == 2) {
+        const int gm = m0 + gr, kq = k0 + 4 * (gq + 2 * i);
+        float v[4];
 #pragma unroll
-      for (int i = 0; i < kPer; ++i) {
-        const int gm = m0 + r0 + 16 * i;
-        float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        if (gm < M && gk < K) v = *reinterpret_cast<const float4*>(X + static_cast<size_t>(gm) * ldx + gk);
-        ra[4 * i] = v.x; ra[4 * i + 1] = v.y; ra[4 * i + 2] = v.z; ra[4 * i + 3] = v.w;
+        for (int e = 0; e < 4; ++e) {
+          const int k = kq + e;
+          v[e] = 0.0f;
+          if (gm < M && k < K)
+            v[e] = fminf(fmaxf(rs.ring[static_cast<size_t>(Kr[k]) * rs.ld + gm], -rs.clip), rs.clip);
+        }
+        ra[i] = make_float4(v[0], v[1], v[2], v[3]);
+      } else {
+        const int gm = m0 + sr + 32 * i, gk = k0 + sc;
+        ra[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (gm < M && gk < K) ra[i] = *reinterpret_cast<const float4*>(X + static_cast<size_t>(gm) * ldx + gk);
       }
-    }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-      const int gn = n0 + r0 + 16 * i;
-      float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      if (gn < N && gk < K) v = *reinterpret_cast<const float4*>(W + static_cast<size_t>(gn) * K + gk);
-      rw[4 * i] = v.x; rw[4 * i + 1] = v.y; rw[4 * i + 2] = v.z; rw[4 * i + 3] = v.w;
+      const int gn = n0 + sr + 32 * i, gk = k0 + sc;
+      rw[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (gn < N && gk < K) rw[i] = *reinterpret_cast<const float4*>(W + static_cast<size_t>(gn) * K + gk);
     }
   };
-  fetch(0);
-  for (int k0 = 0; k0 < K; k0 += kKc) {
-    if (kSrc == 2) {
+  auto stage = [&](int buf) {
+    float* A = lds + buf * 2 * kDbTile;
+    float* B = A + kDbTile;
 #pragma unroll
-      for (int i = 0; i < kGPer; ++i) As[lds_at(tid & 127, kk_r + 2 * i)] = ra[i];
-    } else if (kSrc == 3) {
-#pragma unroll
-      for (int i = 0; i < kPer; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) As[lds_at(m4 + v, kk4 + 8 * i)] = ra[4 * i + v];
-    } else {
-#pragma unroll
-      for (int i = 0; i < kPer; ++i)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) As[lds_at(r0 + 16 * i, c0 + v)] = ra[4 * i + v];
+    for (int i = 0; i < 4; ++i) {
+      if (kSrc == 1) {
+        float* pa = A + fr * kDbStride + fk + 8 * i;
+        pa[0] = ra[i].x;
+        pa[kDbStride] = ra[i].y;
+        pa[2 * kDbStride] = ra[i].z;
+        pa[3 * kDbStride] = ra[i].w;
+      } else {
+        float2* pa = reinterpret_cast<float2*>(kSrc == 2 ? A + gr * kDbStride + 4 * (gq + 2 * i)
+                                                         : A + (sr + 32 * i) * kDbStride + sc);
+        pa[0] = make_float2(ra[i].x, ra[i].y);
+        pa[1] = make_float2(ra[i].z, ra[i].w);
+      }
+      float2* pb = reinterpret_cast<float2*>(B + (sr + 32 * i) * kDbStride + sc);
+      pb[0] = make_float2(rw[i].x, rw[i].y);
+      pb[1] = make_float2(rw[i].z, rw[i].w);
     }
+  };
+  auto compute = [&](int buf, int npair) {
+    const float* A = lds + buf * 2 * kDbTile;
+    const float* ap = A + (wr + col) * kDbStride + 16 * half;
+    const float* bp = A + kDbTile + (wc + col) * kDbStride + 16 * half;
 #pragma unroll
-    for (int i = 0; i < kPer; ++i)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Ws[lds_at(r0 + 16 * i, c0 + v)] = rw[4 * i + v];
-    __syncthreads();
-    if (k0 + kKc < K) fetch(k0 + kKc);
-    if (live) {
-      const float* ap = As + (wr + col) * kPad + 32 * half;
-      const float* bp = Ws + (wc + col) * kPad + 32 * half;
-#pragma unroll
-      for (int kk = 0; kk < kKc; kk += 2) {
-        const float a0 = ap[kk / 2], a1 = ap[32 * kPad + kk / 2];
-        const float b0 = bp[kk / 2], b1 = bp[32 * kPad + kk / 2];
-        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, acc[0][0], 0, 0, 0);
-        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b1, acc[0][1], 0, 0, 0);
-        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b0, acc[1][0], 0, 0, 0);
-        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, acc[1][1], 0, 0, 0);
+    for (int jp = 0; jp < kDbK / 4; ++jp) {          // steps s = 2 jp, 2 jp + 1
+      if (jp < npair) {
+        const float2 a0 = *reinterpret_cast<const float2*>(ap + 2 * jp);
+        const float2 a1 = *reinterpret_cast<const float2*>(ap + 32 * kDbStride + 2 * jp);
+        const float2 b0 = *reinterpret_cast<const float2*>(bp + 2 * jp);
+        const float2 b1 = *reinterpret_cast<const float2*>(bp + 32 * kDbStride + 2 * jp);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b0.x, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.x, b1.x, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b0.x, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.x, b1.x, acc[1][1], 0, 0, 0);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b0.y, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a0.y, b1.y, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b0.y, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x2f32(a1.y, b1.y, acc[1][1], 0, 0, 0);
       }
     }
+  };
+  const int nch = (K + kDbK - 1) / kDbK;
+  fetch(0);
+  stage(0);
+  if (nch > 1) fetch(kDbK);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    // columns 16 + s of half 1 lie past K whenever column s does
+    const int klen = min(kDbK, K - c * kDbK);
+    const int npair = klen >= kDbK / 2 ? kDbK / 4 : (klen + 1) / 2;
+    if (live) compute(c & 1, npair);
+    if (c + 1 < nch) stage((c + 1) & 1);
+    if (c + 2 < nch) fetch((c + 2) * kDbK);
     __syncthreads();
   }
   if (!live) return;
@@ -385,16 +407,16 @@ int launch_linear(const float* X, int ldx, const RingSrc& rs, int M, int K, cons
   // rows of the ring read as float4: needs ld % 4 == 0 (then rows up to round4(M) <= ld exist)
   const bool ring4 = rs.ring && rs.ld % 4 == 0 && aligned16(rs.ring);
   if (M >= kBigM && N >= 256 && wvec && xvec) {
-    const dim3 grid((M + kBigRows - 1) / kBigRows, (N + kBigCols - 1) / kBigCols);
-#define SPI_BIG(E, S) linear_big_kernel<E, S><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
+    const dim3 grid((M + kDbRows - 1) / kDbRows, (N + kDbCols - 1) / kDbCols);
+#define SPI_DB(E, S) linear_db_kernel<E, S><<<grid, 256, 0, stream>>>(X, ldx, rs, M, K, W, b, N, Y, ldy)
     if (ring4) {
-      if (elu) SPI_BIG(true, 3); else SPI_BIG(false, 3);
+      if (elu) SPI_DB(true, 1); else SPI_DB(false, 1);
     } else if (rs.ring) {
-      if (elu) SPI_BIG(true, 2); else SPI_BIG(false, 2);
+      if (elu) SPI_DB(true, 2); else SPI_DB(false, 2);
     } else {
-      if (elu) SPI_BIG(true, 1); else SPI_BIG(false, 1);
+      if (elu) SPI_DB(true, 0); else SPI_DB(false, 0);
     }
-#undef SPI_BIG
+#undef SPI_DB
   } else {
     const dim3 grid((M + 31) / 32, (N + kCols - 1) / kCols);
     const int split = split_for(M, K, N);
