@@ -237,6 +237,54 @@ def test_adjoint_gradient_matches_tangent_and_oracle(gpu, sim, theta0, legs):
         print(method, "gradient of 64 x 4 x H10:", round((time.perf_counter() - t0) * 1e3, 2), "ms")
 
 
+@pytest.mark.parametrize("legs", [True, False])
+def test_adjoint_gradient_at_fisher_shape_matches_oracle(gpu, sim, theta0, legs):
+    """The reverse-mode kernel at the Fisher workload's configuration (H 50, checkpoints every 10
+    control steps: five blocks, each reversed from its checkpoint), four lanes and one lane per
+    rollout: cost and all 25 gradient rows of 4 segments x 3 candidates near the truth against
+    the FP64 oracle (2e-2 of each gradient's largest entry, the bound of the H 10 test), and
+    against the forward-mode kernel on 32 candidates."""
+    import spio
+    from spi_amd.agents.fisher import IDENTIFIED, RolloutCost, _rows, adjoint_cost_grad
+    from spi_amd.agents.sysid import ground_truth, identification_prior
+    from spi_amd.data import synthetic_segments
+    from spi_amd.model import load_model
+    from spi_amd.rollout import RolloutEngine
+    H = 50
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    truth = ground_truth(prior, seed=61)
+    segs = synthetic_segments(eng.simulate_np, truth, sim, n_seg=4, horizon=H, seed=62)
+    seg_dev = eng.upload_segments(segs)
+    rng = np.random.default_rng(63)
+    cand = np.repeat(truth[None], 32, axis=0)
+    cand[1:, prior.free] *= 1.0 + 0.02 * rng.standard_normal((31, int(prior.free.sum())))
+    cand = np.clip(cand, prior.lo, prior.hi)
+    theta = torch.tensor(np.ascontiguousarray(cand.T), dtype=torch.float32, device="cuda")
+    cost_a, grad_a = adjoint_cost_grad(eng, theta, seg_dev, H, checkpoint_every=10, legs=legs)
+    assert torch.isfinite(cost_a).all() and torch.isfinite(grad_a).all()
+    o = spio.Oracle(load_model(), sim)
+    g, ca = grad_a.cpu().numpy(), cost_a.cpu().numpy()
+    for c in range(3):
+        cr, gr = o.cost_grad(cand[c].astype(np.float32).astype(np.float64), np.arange(25), segs.init,
+                             segs.actions, segs.targets, 0, eng.weights)
+        assert np.all(np.abs(ca[:, c] - cr) <= 2e-3 * cr + 1e-4), (c, ca[:, c], cr)
+        sc = np.abs(gr).max()
+        err = np.abs(g[:, c, :] - gr).max() / sc
+        print("candidate", c, "gradient error / scale", err)
+        assert err < 2e-2, (c, err)
+    w = torch.full((4,), 0.25, device="cuda")
+    rows = _rows(IDENTIFIED)
+    ta = theta.clone().requires_grad_(True)
+    RolloutCost.apply(ta, eng, seg_dev, H, w, rows, "adjoint", 10).sum().backward()
+    tt = theta.clone().requires_grad_(True)
+    RolloutCost.apply(tt, eng, seg_dev, H, w, rows, "tangent").sum().backward()
+    ga, gt = ta.grad.cpu().numpy()[rows], tt.grad.cpu().numpy()[rows]
+    err = np.abs(ga - gt) / (np.abs(gt).max(1, keepdims=True) + 1e-12)
+    print("adjoint vs tangent at H 50, max error / row scale:", err.max())
+    assert err.max() < 2e-2
+
+
 @pytest.mark.parametrize("cost_mode", ["tracking", "reference"])
 def test_leg_parallel_adjoint_matches(gpu, sim, theta0, cost_mode):
     """Reverse mode with four lanes per rollout (spi_rollout_cost_adjoint_legs) against one lane
