@@ -34,8 +34,8 @@ METRIC = "env-steps/s, sysID solves/s at ≤1% param error; p50/p99 solve latenc
 FLOPS_PER_CONTROL_STEP = 84690.6
 PEAK_FP32_TFLOPS = 157.3          # MI355X published FP32 (vector) peak
 # HBM bytes per rollout_cost_kernel launch of this workload: rocprofv3 --pmc FETCH_SIZE
-# (2,368.1 KiB, x 2.000 from a 256 MiB copy's reading) + WRITE_SIZE (4,096.0 KiB, x 1.000).
-TRAFFIC_BYTES_PER_LAUNCH = 9044096
+# (2,359.5 KiB, x 2.000 from a 256 MiB copy's reading) + WRITE_SIZE (4,096.0 KiB, x 1.000).
+TRAFFIC_BYTES_PER_LAUNCH = 9026560
 
 N_SAMPLES, N_SEG, HORIZON, ITERS = 4096, 256, 20, 12
 FIXTURE = os.path.join(ROOT, "tests", "golden", "c1_fp64_minimiser.npz")
