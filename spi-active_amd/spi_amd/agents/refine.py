@@ -63,6 +63,7 @@ class GaussNewtonRefiner:
         self.delta = torch.empty((1, self.P), dtype=torch.float64, device=dev)
         self.cost = torch.empty(1, dtype=torch.float64, device=dev)
         self.pred = torch.empty(1, dtype=torch.float64, device=dev)
+        self._gram_theta: Optional[np.ndarray] = None     # the theta self.gram belongs to
 
     def _theta_dev(self, theta: np.ndarray) -> torch.Tensor:
         return torch.from_numpy(np.asarray(theta, np.float32).reshape(P_COUNT, 1).copy()).to(self.engine.device)
@@ -80,22 +81,27 @@ class GaussNewtonRefiner:
         return float(self.objectives(theta)[0])
 
     def step(self, theta: np.ndarray, lam: float):
-        """One damped Gauss-Newton step: (delta [P], cost at theta, predicted cost after it)."""
-        t = self._theta_dev(theta)
+        """One damped Gauss-Newton step: (delta [P], cost at theta, predicted cost after it).
+        The residual Jacobian and [J r]^T [J r] are kept for the last theta: a step retried at
+        the same point with another damping only re-solves the normal equations."""
         L = _lib.lib()
-        rc = getattr(L, self.jacobian_entry)(C_ref(self.engine.args), C_ref(self.cargs), C_ref(self.spec),
-                                             _lib.ptr(t), 1, 1, _lib.ptr(self.segments), _lib.ptr(self.seg_w),
-                                             self.n_seg, self.H, _lib.ptr(self.Jr), _lib.ptr(self.status, torch.int32),
-                                             _lib.stream())
-        _lib.check(rc, self.jacobian_entry)
-        _lib.check(L.spi_fim_assemble(_lib.ptr(self.Jr), self.n_seg, self.H * 37, self.P + 1, _lib.ptr(self.gram),
-                                      _lib.stream()), "spi_fim_assemble")
+        if self._gram_theta is None or not np.array_equal(self._gram_theta, theta):
+            t = self._theta_dev(theta)
+            rc = getattr(L, self.jacobian_entry)(C_ref(self.engine.args), C_ref(self.cargs), C_ref(self.spec),
+                                                 _lib.ptr(t), 1, 1, _lib.ptr(self.segments), _lib.ptr(self.seg_w),
+                                                 self.n_seg, self.H, _lib.ptr(self.Jr),
+                                                 _lib.ptr(self.status, torch.int32), _lib.stream())
+            _lib.check(rc, self.jacobian_entry)
+            _lib.check(L.spi_fim_assemble(_lib.ptr(self.Jr), self.n_seg, self.H * 37, self.P + 1,
+                                          _lib.ptr(self.gram), _lib.stream()), "spi_fim_assemble")
+            self._gram_theta = np.array(theta, np.float64, copy=True)
         self.lam.fill_(lam)
         rc = L.spi_gn_step(_lib.ptr(self.gram), _lib.ptr(self.status, torch.int32), 1, self.n_seg, self.P,
                            _lib.ptr(self.lam, torch.float64), _lib.ptr(self.delta, torch.float64),
                            _lib.ptr(self.cost, torch.float64), _lib.ptr(self.pred, torch.float64), _lib.stream())
         _lib.check(rc, "spi_gn_step")
-        return self.delta[0].cpu().numpy(), float(self.cost.item()), float(self.pred.item())
+        out = torch.cat([self.delta[0], self.cost, self.pred]).cpu().numpy()      # one read-back
+        return out[:self.P], float(out[self.P]), float(out[self.P + 1])
 
     def refine(self, theta0: np.ndarray, iterations: int = 10, lam0: float = 1e-3, tol: float = 1e-10,
                line_search: int = 4) -> RefineResult:
