@@ -115,6 +115,7 @@ def main():
         same_seed = "theta_fp64_same_seed" in fx.files
         theta64 = fx["theta_fp64_same_seed"] if same_seed else fx["theta_fp64"]
         cost64 = float(fx["cost_fp64_same_seed"]) if same_seed else cost_indep
+        cem64 = (fx["cem_means_same_seed"][-1], fx["cem_stds_same_seed"][-1]) if same_seed else None
     segs = SegmentSet.from_packed(seg_np, HORIZON)
     eng = RolloutEngine(sim)
     seg_dev = torch.from_numpy(seg_np).to(dev)
@@ -214,6 +215,11 @@ def main():
         "solve": "CEM 12 iterations + LM %d iterations" % args.refine if refiner is not None else "CEM 12 iterations",
         "per_param": per64,
         "cem_only_per_param": per_cem64,
+        # the CEM alone against the FP64 CEM of the same seed: the mean's distance in units of
+        # the FP64 search's final width (population std), per free parameter, at most
+        "cem_mean_vs_fp64_cem_in_std": (round(float(np.max(np.abs(cem_mean.cpu().numpy().astype(np.float64)
+                                                                   - cem64[0])[free] / cem64[1][free])), 4)
+                                        if cem64 is not None else None),
         "se_rel_fp64": {PARAM_NAMES[i]: float("%.3g" % se_rel[i]) for i in range(len(se_rel)) if free[i]},
     }
     refined_rec = None
