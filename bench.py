@@ -116,6 +116,10 @@ def main():
         theta64 = fx["theta_fp64_same_seed"] if same_seed else fx["theta_fp64"]
         cost64 = float(fx["cost_fp64_same_seed"]) if same_seed else cost_indep
         cem64 = (fx["cem_means_same_seed"][-1], fx["cem_stds_same_seed"][-1]) if same_seed else None
+        # spread of the FP64 local minimisers reached from independent starts
+        starts = [fx["theta_fp64_truth_start"], fx["theta_fp64_cem_start"]] + ([theta64] if same_seed else [])
+        pts = np.stack(starts)
+        spread64 = (pts.max(0) - pts.min(0)) / np.maximum(np.abs(pts).min(0), 1e-12)
     segs = SegmentSet.from_packed(seg_np, HORIZON)
     eng = RolloutEngine(sim)
     seg_dev = torch.from_numpy(seg_np).to(dev)
@@ -187,7 +191,12 @@ def main():
     value = args.steps * ctrl_per_solve / total
 
     free = prior.free
-    ident = [i for i in range(len(free)) if free[i] and 2 * se_rel[i] < 0.01]
+    # determined to 1 % by the data: twice the asymptotic standard error of the FP64 minimiser
+    # below 1 %, and the FP64 local minimisers from independent starts within 1 % of each other
+    # (the objective is piecewise smooth; its creases hold local minima the standard error
+    # does not see)
+    crb = [i for i in range(len(free)) if free[i] and 2 * se_rel[i] < 0.01]
+    ident = [i for i in crb if spread64[i] < 0.01]
 
     def errors(est, ref):
         rel = np.abs(np.asarray(est, np.float64) - ref) / np.maximum(np.abs(ref), 1e-12)
@@ -209,9 +218,13 @@ def main():
         "independent_fp64_search_cost": cost_indep,
         "device_cost_at_answer": (round(res.cost, 7) if res is not None else None),
         "identifiable": [PARAM_NAMES[i] for i in ident],
-        "identifiable_rule": "twice the asymptotic standard error of the FP64 minimiser < 1 % of its value",
+        "identifiable_rule": ("twice the asymptotic standard error of the FP64 minimiser < 1 % of its value, "
+                              "and the FP64 local minimisers from independent starts within 1 % of each other"),
         "within_1pct": within, "all_identifiable_within_1pct": len(within) == len(ident),
         "max_rel_identifiable": round(float(max(rel64[i] for i in ident)), 5),
+        "standard_error_rule_only": {PARAM_NAMES[i]: {"rel_err": round(float(rel64[i]), 5),
+                                                      "fp64_minima_spread": round(float(spread64[i]), 5)}
+                                     for i in crb if i not in ident},
         "solve": "CEM 12 iterations + LM %d iterations" % args.refine if refiner is not None else "CEM 12 iterations",
         "per_param": per64,
         "cem_only_per_param": per_cem64,
