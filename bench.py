@@ -13,7 +13,8 @@ Workload (one "step" = one full sysID solve, inputs resident in HBM):
   Per solve: 4096 * 256 * 20 * 12 = 251,658,240 control steps.
 
 Prints one JSON line (metric, value = control steps per second over the timed solves); extra
-keys carry the 1-D landscape's latency and the Fisher objective's time (short runs).
+keys carry the other configurations in short runs: the 1-D landscape's latency, one batch of
+64 uneven problems, a 2,000-request serving stream and the Fisher objective's time.
 """
 import argparse
 import json
@@ -49,7 +50,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-candidates", type=int, default=256, help="oracle CPU engine: candidates timed")
     p.add_argument("--no-plain-baseline", action="store_true")
-    p.add_argument("--no-extras", action="store_true", help="skip the landscape and Fisher keys")
+    p.add_argument("--no-extras", action="store_true",
+                   help="skip the landscape, batched-problems, serving and Fisher keys")
     p.add_argument("--refine", type=int, default=20,
                    help="Levenberg-Marquardt iterations after CEM, inside each timed solve (0: CEM only)")
     p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve",
@@ -272,6 +274,20 @@ def main():
                                                   "argmin_mass_kg": ls["argmin_mass_kg"]}
         except Exception as e:  # reported, never fatal
             extras["landscape_mass_256x64_h5"] = {"failed": str(e)}
+        try:
+            b64 = workloads.batch64(steps=1, warmup=0, autotune=False)
+            extras["batch64_uneven"] = {"seconds": round(b64["ms_per_step"] / 1e3, 3), "env_steps_per_s": b64["value"],
+                                        "solves_per_s": b64["solves_per_s"],
+                                        "control_steps_per_batch": b64["config"]["control_steps_per_batch"]}
+        except Exception as e:
+            extras["batch64_uneven"] = {"failed": str(e)}
+        try:
+            sv = workloads.serve(n_requests=2000)
+            extras["serve_poisson_200hz_2000"] = {"jobs_per_s": sv["value"], "latency_ms": sv["latency_ms"],
+                                                  "sysid_solves_per_s": sv["sysid_solves_per_s"],
+                                                  "max_active": sv["max_active"]}
+        except Exception as e:
+            extras["serve_poisson_200hz_2000"] = {"failed": str(e)}
         try:
             fi = workloads.fisher(steps=1, warmup=1)
             extras["fisher_22x4096x128_h50"] = {"objective_ms": fi["ms_per_step"], "phase_ms": fi["phase_ms"],
