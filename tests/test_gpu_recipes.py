@@ -1,0 +1,68 @@
+"""The reference's data recipes (scripts/data/sine.py, jump.py, stand.py) recorded with the device
+simulator, and the identification they feed.
+
+* Each recipe recorded on the device follows the same recipe recorded by the FP64 oracle over
+  its first half second (25 control steps; measured 1e-6 - 3e-6 rad in the joints, 2e-7 m in the
+  base position).
+* A 5 s sine recording made on the device under a seeded truth, cut into the reference's
+  sliding windows of 20 steps (scripts/eval.py:101-171, through the npz schema), gives the same
+  base-mass landscape argmin on the device as in the FP64 oracle (65 masses on [5, 9] kg), the
+  plain method of scripts/mass_landscape.py:111-171 end to end."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_sim(o):
+    def simulate(rows, init, acts):
+        return np.stack([o.simulate(np.asarray(r, np.float64), i0, a)[0] for r, i0, a in zip(rows, init, acts)])
+    return simulate
+
+
+def test_recipes_follow_oracle(gpu, model, sim, theta0):
+    import spio
+    from spi_amd.data import RECIPES, record_recipe, standing_state
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    o = spio.Oracle(model, sim)
+    init = standing_state(0.30)
+    for name in RECIPES:
+        sd, ad = record_recipe(eng.simulate_np, theta0, sim, name, init, duration=0.5)
+        so, ao = record_recipe(_oracle_sim(o), theta0, sim, name, init, duration=0.5)
+        assert np.array_equal(ad, ao) and sd.shape == (25, 37)
+        dq = np.abs(sd[:, 13:25] - so[:, 13:25]).max()
+        dp = np.abs(sd[:, 0:3] - so[:, 0:3]).max()
+        print(f"{name}: max |q_device - q_oracle| {dq:.2e} rad, max |pos| {dp:.2e} m over 25 control steps")
+        assert dq < 1e-4 and dp < 1e-5, (name, dq, dp)
+
+
+def test_sine_recording_identifies_mass_like_oracle(gpu, model, sim, theta0, tmp_path):
+    import spio
+    from spi_amd.agents.sysid import mass_landscape
+    from spi_amd.data import load_npz_dataset, record_recipe, save_npz_trajectory, standing_state
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    truth = np.asarray(theta0, np.float64).copy()
+    truth[0] *= 1.07                                     # the robot to identify: 7 % heavier base
+    states, acts = record_recipe(eng.simulate_np, truth, sim, "sine", standing_state(0.30), duration=5.0)
+    f = str(tmp_path / "sine.npz")
+    save_npz_trajectory(f, states, acts, sim.dt * sim.decimation, float(sim.kp[0]), float(sim.kd[0]))
+    H = 20
+    segs = load_npz_dataset([f], H)
+    masses = np.linspace(5.0, 9.0, 65)
+    gc, gi = mass_landscape(eng, eng.upload_segments(segs), H, theta0, masses)
+    rows = np.repeat(np.asarray(theta0, np.float64)[None], masses.size, axis=0)
+    rows[:, 0] = masses
+    rows = rows.astype(np.float32).astype(np.float64)
+    o = spio.Oracle(model, sim)
+    oc = o.rollout_cost_par(rows, segs.init, segs.actions, segs.targets, 0, np.asarray(eng.weights)).mean(axis=1)
+    oi = int(np.argmin(oc))
+    order = np.argsort(oc)
+    tie = oc[order[1]] - oc[order[0]] <= 1e-5 * oc[order[0]]
+    print(f"{segs.n_seg} windows: argmin device {masses[gi]:.4f} kg, oracle {masses[oi]:.4f} kg, truth {truth[0]:.4f} kg")
+    assert gi == oi or (tie and gi == order[1]), (gi, oi)
+    # noise-free data, but the recording and the windows' replay start from different states
+    # (the reference's pairing of frame t with action t): within two grid steps of the truth
+    assert abs(masses[oi] - truth[0]) <= 2 * (masses[1] - masses[0])
