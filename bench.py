@@ -2,7 +2,7 @@
 """Benchmark: 22-parameter CEM system identification of the Go2 on one MI355X.
 
 Workload (one "step" = one full sysID solve, inputs resident in HBM):
-  4096 parameter samples / iteration, 410 elites (ceil 10 %), 12 iterations, then 20
+  4096 parameter samples / iteration, 410 elites (ceil 10 %), 12 iterations, then 10
   Levenberg-Marquardt iterations with exact residual Jacobians; 256 segments of horizon 20
   control steps (dt 0.005 s, decimation 4, 4 integration sub-intervals per physics step),
   PD Kp 20 / Kd 0.5, penalty contact k 5e4 N/m with saturating damping c 100 N s/m, mu 0.8.  The segments are the
@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--no-plain-baseline", action="store_true")
     p.add_argument("--no-extras", action="store_true",
                    help="skip the landscape, batched-problems, serving and Fisher keys")
-    p.add_argument("--refine", type=int, default=20,
+    p.add_argument("--refine", type=int, default=10,
                    help="Levenberg-Marquardt iterations after CEM, inside each timed solve (0: CEM only)")
     p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve",
                                                     "policy"],
