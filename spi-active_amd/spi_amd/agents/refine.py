@@ -18,7 +18,7 @@ import torch
 
 from .. import _lib
 from ..data import segment_stride
-from ..params import PARAM_NAMES, P_COUNT
+from ..params import P_COUNT
 from ..rollout import C_ref, RolloutEngine
 from .fisher import IDENTIFIED, _rows
 

@@ -25,7 +25,7 @@ from .agents.batch import BatchCEM, ProblemSpec
 from .agents.fisher import FIM_ROWS, FisherObjective, fourier_excitations
 from .agents.sysid import SysIdProblem, ground_truth, identification_prior
 from .data import standing_state, synthetic_segments
-from .params import P_MASS, PARAM_NAMES, SimParams, nominal_params
+from .params import P_MASS, SimParams, nominal_params
 from .model import load_model
 from .rollout import RolloutEngine
 

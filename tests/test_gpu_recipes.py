@@ -10,7 +10,6 @@ simulator, and the identification they feed.
   plain method of scripts/mass_landscape.py:111-171 end to end."""
 import numpy as np
 import pytest
-import torch
 
 pytestmark = pytest.mark.gpu
 
