@@ -240,7 +240,7 @@ def main():
     refined_rec = None
     if res is not None:
         refined_rec = {"iterations": res.iterations, "accepted": res.accepted, "cost_before": res.history[0],
-                       "cost_after": res.cost}
+                       "cost_after": res.cost, "line_search_point_taken": res.taken}
 
     launch_ctrl = N_SAMPLES * N_SEG * HORIZON
     kern_s = float(kern.mean()) / 1e3

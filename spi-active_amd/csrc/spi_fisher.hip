@@ -196,52 +196,65 @@ __global__ void fourier_kernel(const float* __restrict__ coeff, int n_exc, int n
 // Gauss-Newton / Levenberg-Marquardt step per candidate from the augmented Gram matrices
 // [J r]^T [J r] of its n_seg segment groups: FP64 sums A = J^T J, b = J^T r, rr = r^T r, then
 // (A + lambda diag(A) + 1e-12 tr(A)/P I) delta = -b by FP64 Cholesky; pred = rr - model(delta).
+// Every sum runs in a fixed order (segments in index order, k ascending), whichever thread
+// computes it: the segment sums keep kGnBatch loads in flight per thread, and in Cholesky
+// column j thread i > j computes L[i][j].
+constexpr int kGnBatch = 16;
 __global__ __launch_bounds__(256) void gn_step_kernel(const float* __restrict__ Gm, const int* __restrict__ status,
                                                       int n_seg, int P, const double* __restrict__ lambda,
                                                       double* __restrict__ delta, double* __restrict__ cost,
                                                       double* __restrict__ pred) {
   __shared__ double M[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
   __shared__ double Ls[SPI_MAX_TANGENTS * SPI_MAX_TANGENTS];
-  __shared__ int bad;
-  const int c = blockIdx.x, L = P + 1;
+  __shared__ double y[SPI_MAX_TANGENTS];
+  const int c = blockIdx.x, L = P + 1, tid = threadIdx.x;
   const size_t g0 = static_cast<size_t>(c) * n_seg;
-  if (threadIdx.x == 0) {
-    int b = 0;
-    for (int s = 0; s < n_seg; ++s) b |= status[g0 + s];
-    bad = b;
-  }
-  for (int e = threadIdx.x; e < L * L; e += blockDim.x) {
+  int b = 0;
+  for (int s = tid; s < n_seg; s += blockDim.x) b |= status[g0 + s];
+  const bool bad = __syncthreads_or(b) != 0;
+  const size_t LL = static_cast<size_t>(L) * L;
+  for (int e = tid; e < L * L; e += blockDim.x) {
+    const float* p = Gm + g0 * LL + e;
     double acc = 0.0;
-    for (int s = 0; s < n_seg; ++s) acc += static_cast<double>(Gm[(g0 + s) * L * L + e]);
+    int s = 0;
+    for (; s + kGnBatch <= n_seg; s += kGnBatch) {
+      float v[kGnBatch];
+#pragma unroll
+      for (int u = 0; u < kGnBatch; ++u) v[u] = p[static_cast<size_t>(s + u) * LL];
+#pragma unroll
+      for (int u = 0; u < kGnBatch; ++u) acc += static_cast<double>(v[u]);
+    }
+    for (; s < n_seg; ++s) acc += static_cast<double>(p[static_cast<size_t>(s) * LL]);
     M[e] = acc;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
   const double rr = M[P * L + P];
-  cost[c] = bad ? INFINITY : rr;
   double tr = 0.0;
   for (int i = 0; i < P; ++i) tr += M[i * L + i];
   const double lam = lambda[c], eps = 1e-12 * tr / P;
   bool pd = !bad;
   for (int j = 0; j < P && pd; ++j) {          // Cholesky of the damped normal matrix (lower)
-    double d = M[j * L + j] * (1.0 + lam) + eps;
+    double d = M[j * L + j] * (1.0 + lam) + eps;   // every thread: the same bits, a uniform test
     for (int k = 0; k < j; ++k) d -= Ls[j * P + k] * Ls[j * P + k];
     if (!(d > 0.0)) { pd = false; break; }
     const double ljj = sqrt(d);
-    Ls[j * P + j] = ljj;
-    for (int i = j + 1; i < P; ++i) {
+    if (tid == j) Ls[j * P + j] = ljj;
+    if (tid > j && tid < P) {
+      const int i = tid;
       double v = 0.5 * (M[i * L + j] + M[j * L + i]);
       for (int k = 0; k < j; ++k) v -= Ls[i * P + k] * Ls[j * P + k];
       Ls[i * P + j] = v / ljj;
     }
+    __syncthreads();
   }
+  if (tid != 0) return;
+  cost[c] = bad ? INFINITY : rr;
   double* dc = delta + static_cast<size_t>(c) * P;
   if (!pd) {
     for (int i = 0; i < P; ++i) dc[i] = 0.0;
     pred[c] = bad ? INFINITY : rr;
     return;
   }
-  double y[SPI_MAX_TANGENTS];
   for (int i = 0; i < P; ++i) {                // L y = -b
     double v = -0.5 * (M[i * L + P] + M[P * L + i]);
     for (int k = 0; k < i; ++k) v -= Ls[i * P + k] * y[k];

@@ -187,3 +187,28 @@ def test_cem_and_refinement_find_fp64_oracle_minimiser(gpu, model, sim):
     # the two land on the same stationary point: every parameter, the poorly determined ones
     # included, within 2 % (measured: 1.9e-3 at most, mass 3e-5)
     assert err.max() <= 2e-2, {PARAM_NAMES[rows[k]]: float(err[k]) for k in range(len(rows)) if err[k] > 2e-2}
+
+
+def test_speculative_jacobians_leave_the_iterates_unchanged(gpu, sim, theta0):
+    """Jacobians started early at the full and half step (beside the line search on a second
+    stream) give bit for bit the iterates of the plain loop, whichever line-search point wins,
+    and a rejected step followed by a retry reuses the current point's Jacobian."""
+    from spi_amd.agents.refine import GaussNewtonRefiner
+    from spi_amd.agents.sysid import identification_prior
+    from spi_amd.rollout import RolloutEngine
+    eng = RolloutEngine(sim)
+    prior = identification_prior(theta0)
+    truth, segs = _data(eng, prior, 0.01, 24, 10, 31)
+    seg_dev = eng.upload_segments(segs)
+    rng = np.random.default_rng(4)
+    start = truth.copy()
+    start[:22] *= 1 + 0.03 * rng.normal(size=22)
+    runs = {}
+    for spec in (False, True):
+        ref = GaussNewtonRefiner(eng, seg_dev, 10, lo=prior.lo, hi=prior.hi)
+        runs[spec] = ref.refine(start, iterations=12, speculate=spec)
+    a, b = runs[False], runs[True]
+    print("taken:", b.taken, "accepted", b.accepted, "of", b.iterations, "history", b.history)
+    assert a.accepted >= 3 and a.history[-1] < a.history[0]
+    assert a.taken == b.taken and a.iterations == b.iterations
+    assert np.array_equal(a.theta, b.theta) and a.history == b.history
