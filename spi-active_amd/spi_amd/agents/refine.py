@@ -93,6 +93,14 @@ class GaussNewtonRefiner:
         _lib.check(L.spi_fim_assemble(_lib.ptr(S["Jr"]), k * self.n_seg, self.H * 37, self.P + 1,
                                       _lib.ptr(S["gram"]), _lib.stream()), "spi_fim_assemble")
 
+    def _start_at(self, theta: np.ndarray) -> None:
+        """Make theta the current point: its Jacobian and Gram into the set the current point
+        does not use (any speculation there is dropped)."""
+        b = 0 if self._cur is None else 1 - self._cur[1]
+        self._jacobians(b, np.asarray(theta, np.float64)[None])
+        self._cur = (np.array(theta, np.float64, copy=True), b, 0)
+        self._spec = None
+
     def speculate(self, thetas: np.ndarray) -> None:
         """Start the Jacobians of up to N_SPECULATE likely next points (not waited for): a later
         step() at one of them finds its Gram ready.  The current point's buffers are untouched."""
@@ -126,11 +134,9 @@ class GaussNewtonRefiner:
                 hit = next((k for k, th in enumerate(self._spec[1]) if np.array_equal(th, theta)), None)
             if hit is not None:
                 self._cur = (theta.copy(), self._spec[0], hit)
+                self._spec = None
             else:
-                b = 0 if self._cur is None else 1 - self._cur[1]
-                self._jacobians(b, theta[None])
-                self._cur = (theta.copy(), b, 0)
-            self._spec = None
+                self._start_at(theta)
         _, b, k = self._cur
         S, g0 = self._sets[b], k * self.n_seg
         self.lam.fill_(lam)
@@ -152,9 +158,17 @@ class GaussNewtonRefiner:
 
         With speculate, the Jacobians at the full and the half step are started before the
         line search, which runs beside them on a second stream; when one of the two wins, the
-        next iteration's Jacobian is already there.  The iterates are the same either way."""
+        next iteration's Jacobian is already there.  The first Jacobian likewise runs beside the
+        cost at theta0.  The iterates are the same either way."""
         theta = np.asarray(theta0, np.float64).copy()
-        cost = self.objective(theta)
+        self._cur = self._spec = None          # every refinement computes its own Jacobians
+        if speculate:
+            # the first Jacobian and the cost at theta0 side by side
+            self._start_at(theta)
+            with torch.cuda.stream(self._side):
+                cost = self.objective(theta)
+        else:
+            cost = self.objective(theta)
         res = RefineResult(theta.copy(), cost, [cost])
         lam = lam0
         fracs = 0.5 ** np.arange(line_search + 1)
@@ -164,8 +178,9 @@ class GaussNewtonRefiner:
             cands[:, self.rows] += fracs[:, None] * delta[None, :]
             if self.lo is not None:
                 cands = np.clip(cands, self.lo, self.hi)
-            if speculate:
+            if speculate and it + 1 < iterations:     # the last iteration has no next step
                 self.speculate(cands)
+            if speculate:
                 # the line search reads only the segments, the weights and host data, none of
                 # which the main stream is writing: no ordering between the two streams
                 with torch.cuda.stream(self._side):
