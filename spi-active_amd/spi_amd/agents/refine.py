@@ -163,6 +163,9 @@ class GaussNewtonRefiner:
         theta = np.asarray(theta0, np.float64).copy()
         self._cur = self._spec = None          # every refinement computes its own Jacobians
         if speculate:
+            # the side stream sees what the caller's stream wrote so far (the segments, the
+            # weights); nothing that this refinement enqueues later
+            self._side.wait_stream(torch.cuda.current_stream())
             # the first Jacobian and the cost at theta0 side by side
             self._start_at(theta)
             with torch.cuda.stream(self._side):
@@ -182,7 +185,7 @@ class GaussNewtonRefiner:
                 self.speculate(cands)
             if speculate:
                 # the line search reads only the segments, the weights and host data, none of
-                # which the main stream is writing: no ordering between the two streams
+                # which the caller's stream writes during the refinement
                 with torch.cuda.stream(self._side):
                     c_new = self.objectives(cands)
             else:
