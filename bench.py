@@ -54,10 +54,12 @@ def parse():
                    help="skip the landscape, batched-problems, serving and Fisher keys")
     p.add_argument("--refine", type=int, default=10,
                    help="Levenberg-Marquardt iterations after CEM, inside each timed solve (0: CEM only)")
-    p.add_argument("--workload", default="cem22", choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve",
-                                                    "policy"],
+    p.add_argument("--workload", default="cem22",
+                   choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve", "store", "commands",
+                            "policy"],
                    help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
-                        "64 batched uneven problems, Fisher scoring of excitations and the serving loop")
+                        "64 batched uneven problems, Fisher scoring of excitations, the gradient, the serving "
+                        "loop, dataset windows, command sequences and the policy-in-the-loop rollout")
     p.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
     p.add_argument("--policy-batches", default="1024,8192",
                    help="policy: trials per iteration at the reference batch[,wide batch]")

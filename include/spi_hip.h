@@ -381,6 +381,13 @@ int spi_command_sequences(const SpiCommandSpec* spec, const float* params_dev, i
  * scripts/eval.py:101-171 (one window per start index, stride 1). */
 int spi_pack_windows(const float* states_dev, const float* actions_dev, int n_steps, int horizon, int first,
                      int n_windows, float* segments_dev, hipStream_t stream);
+/* spi_pack_windows over many trajectories held back to back in one buffer, in one launch:
+ * window w starts at row rows_dev[w] of states_dev [n_rows][37] / actions_dev [n_rows][12]
+ * (the caller's table keeps each window inside its own trajectory).  A window whose rows
+ * r .. r + horizon do not all lie below n_rows is written as NaN; nothing outside the buffers
+ * is read.  The device side of the loop over recorded files in scripts/eval.py:101-171. */
+int spi_pack_windows_rows(const float* states_dev, const float* actions_dev, int n_rows, const int* rows_dev,
+                          int n_windows, int horizon, float* segments_dev, hipStream_t stream);
 
 /* Tracking cost and its exact gradient with respect to all 25 parameter rows for every
  * (segment, candidate) pair, by reverse mode (adjoint) with checkpoints every

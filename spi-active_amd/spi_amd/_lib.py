@@ -140,6 +140,7 @@ def lib() -> C.CDLL:
         "spi_gn_step": [vp, vp, ip, ip, ip, vp, vp, vp, vp, vp],
         "spi_command_sequences": [vp, vp, ip, ip, vp, vp],
         "spi_pack_windows": [vp, vp, ip, ip, ip, ip, vp, vp],
+        "spi_pack_windows_rows": [vp, vp, ip, vp, ip, ip, vp, vp],
         "spi_rollout_cost_adjoint": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
         "spi_rollout_cost_adjoint_legs": [sa, ca, vp, ip, ip, vp, ip, ip, ip, vp, C.c_longlong, vp, vp, vp],
         "spi_policy_step": [sa, vp, ip, ip, ip, vp, vp, vp, vp, ip, vp, vp, vp],

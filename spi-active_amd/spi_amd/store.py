@@ -4,7 +4,9 @@
 host (init at t, actions t..t+H-1, target at t+H) and copies the arrays to the device.  The
 store keeps each trajectory once in HBM (``[T, 37]`` states in the IsaacGym root layout +
 ``[T, 12]`` actions) and packs the windows for any horizon on the device
-(``spi_pack_windows``), so the host never holds the H-fold expanded windows.  Trajectories
+(``spi_pack_windows_rows``: every window of every chosen trajectory in one launch, from a
+back-to-back copy of the trajectories made on the first call after a trajectory was added),
+so the host never holds the H-fold expanded windows.  Trajectories
 come from npz files (scripts/data/common.py:83-101 keys), real-robot logs in the same schema,
 or arrays.
 """
@@ -56,6 +58,15 @@ class SegmentStore:
         _lib.require_gpu()
         self.device = torch.device(device)
         self.trajectories: List[Trajectory] = []
+        self._joined = None           # (states [R, 37], actions [R, 12], first row of each trajectory)
+
+    def _joined_buffers(self):
+        """All trajectories back to back in one device buffer (rebuilt after an add)."""
+        if self._joined is None or len(self._joined[2]) != len(self.trajectories):
+            starts = np.cumsum([0] + [tr.n_steps for tr in self.trajectories])[:-1]
+            self._joined = (torch.cat([tr.states for tr in self.trajectories]),
+                            torch.cat([tr.actions for tr in self.trajectories]), starts)
+        return self._joined
 
     def add_arrays(self, states: np.ndarray, actions: np.ndarray, kp: float = 20.0, kd: float = 0.5,
                    name: Optional[str] = None) -> int:
@@ -87,17 +98,23 @@ class SegmentStore:
         ends = np.zeros(n, bool)
         src, start, kp, kd = (np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n, np.float32),
                               np.zeros(n, np.float32))
+        states, actions, first = self._joined_buffers()
+        if states.shape[0] >= 2 ** 31:
+            raise ValueError("the store holds more than 2^31 - 1 trajectory rows")
+        rows = np.zeros(n, np.int32)
         o = 0
-        L = _lib.lib()
         for i, c in zip(which, counts):
             if c == 0:
                 continue
             tr = self.trajectories[i]
-            rc = L.spi_pack_windows(_lib.ptr(tr.states), _lib.ptr(tr.actions), tr.n_steps, horizon, 0, c,
-                                    _lib.ptr(out[o:o + c]), _lib.stream())
-            _lib.check(rc, "spi_pack_windows")
             ends[o + c - 1] = True
             src[o:o + c], start[o:o + c] = i, np.arange(c)
+            rows[o:o + c] = first[i] + np.arange(c)
             kp[o:o + c], kd[o:o + c] = tr.kp, tr.kd
             o += c
+        rows_dev = torch.from_numpy(rows).to(self.device)
+        rc = _lib.lib().spi_pack_windows_rows(_lib.ptr(states), _lib.ptr(actions), states.shape[0],
+                                              _lib.ptr(rows_dev, torch.int32), n, horizon, _lib.ptr(out),
+                                              _lib.stream())
+        _lib.check(rc, "spi_pack_windows_rows")
         return SegmentBatch(out, horizon, ends, src, start, kp, kd)

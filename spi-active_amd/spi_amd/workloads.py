@@ -10,6 +10,8 @@
   matrix cores, FP64 logdet per excitation.
 * ``serve`` -- Poisson stream of sysID and excitation-scoring jobs through the
   continuous-batching server under an HBM budget.
+* ``store`` -- dataset windows: 64 trajectories x 2500 steps packed into H 20 segment records.
+* ``commands`` -- command sequences of one 1024-trial ask batch in the three sampling modes.
 
 Each returns the dict ``bench.py`` prints as its JSON line.
 """
@@ -438,3 +440,105 @@ def policy(steps: int, warmup: int, batch: int = 1024, wide_batch: int = 8192, t
             "dtype": "fp32", "data": "synthetic (random actor weights)",
             "config": {"workload": "policy_active_sysid", "explore": ["mass"], "total_steps": total_steps,
                        "command_mode": "constant", "ksync_steps": 5, "obs_layout": obs_layout}, **res}
+
+
+# ------------------------------------------------------------------ dataset store and commands
+def store(steps: int, warmup: int, n_traj: int = 64, n_steps: int = 2500, horizon: int = 20) -> Dict:
+    """Dataset store (scripts/eval.py:101-171 on the device): every stride-1 window of 64
+    recorded trajectories of 2500 control steps (50 s at 50 Hz) packed into H 20 segment records
+    by ``spi_pack_windows_rows`` in one launch (timed with HIP events), and the store's own
+    ``windows`` call around it (host bookkeeping and the row table's upload included).
+    Synthetic trajectories (seeded normal values: the copy does not look at them).  Algorithmic
+    bytes: the windows written once and the trajectories read once."""
+    from . import _lib
+    from .data import segment_stride
+    from .store import SegmentStore
+    rng = np.random.default_rng(0)
+    st = SegmentStore()
+    for _ in range(n_traj):
+        st.add_arrays(rng.normal(size=(n_steps, 37)).astype(np.float32),
+                      rng.normal(size=(n_steps, 12)).astype(np.float32))
+    stride = segment_stride(horizon)
+    per = n_steps - horizon
+    n_win = n_traj * per
+    out = torch.empty((n_win, stride), dtype=torch.float32, device="cuda")
+    states, actions, first = st._joined_buffers()
+    rows = torch.from_numpy((first[:, None] + np.arange(per)[None, :]).astype(np.int32).ravel()).cuda()
+    L = _lib.lib()
+    stream = torch.cuda.current_stream()
+    kern_ms, wall_ms = [], []
+    for rep in range(warmup + steps):
+        a0, a1 = _events(stream)
+        a0.record(stream)
+        _lib.check(L.spi_pack_windows_rows(_lib.ptr(states), _lib.ptr(actions), states.shape[0],
+                                           _lib.ptr(rows, torch.int32), n_win, horizon, _lib.ptr(out),
+                                           _lib.stream()), "spi_pack_windows_rows")
+        a1.record(stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch = st.windows(horizon)                 # the store's own call, with its host bookkeeping
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if rep >= warmup:
+            kern_ms.append(a0.elapsed_time(a1))
+            wall_ms.append((t1 - t0) * 1e3)
+    assert torch.equal(batch.segments, out)
+    written = 4.0 * n_win * stride
+    read = 4.0 * n_traj * n_steps * (37 + 12)
+    k_s = float(np.mean(kern_ms)) / 1e3
+    gbs = (written + read) / k_s / 1e9
+    return {"metric": "windows packed/s", "value": round(n_win / k_s, 1), "unit": "windows/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": round(float(np.mean(kern_ms)), 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (copy)",
+            "data": "synthetic",
+            "config": {"workload": f"store_{n_traj}x{n_steps}_h{horizon}", "trajectories": n_traj,
+                       "steps_per_trajectory": n_steps, "horizon": horizon, "windows": n_win,
+                       "segment_floats": stride},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "algorithmic_bytes": written + read},
+            "store_windows_call_ms": round(float(np.mean(wall_ms)), 3)}
+
+
+def commands(steps: int, warmup: int, n_trials: int = 1024) -> Dict:
+    """Command sequences of the active-exploration optimizer (active_sysid.py:295-400 on the
+    device): one ask batch of 1024 trials of the reference's command block
+    (algo/active_sysid.yaml: 14 command dimensions, 3 sampled, 5 updates of 250 control steps),
+    in each of the three sampling modes.  Algorithmic bytes: the sequences written once and the
+    trial parameters read once."""
+    import os
+    import yaml
+    from .agents.commands import CommandGenerator
+    path = os.path.join(os.path.dirname(__file__), "configs", "algo", "active_sysid.yaml")
+    with open(path) as f:
+        cmd = yaml.safe_load(f)["algo"]["config"]["command"]
+    dt = SimParams().dt * SimParams().decimation
+    stream = torch.cuda.current_stream()
+    rng = np.random.default_rng(0)
+    modes = {}
+    for mode in ("constant", "polynomial", "bezier"):
+        gen = CommandGenerator(cmd["command_ranges"], cmd["default_command"], cmd["command_sampling_idxs"], mode,
+                               float(cmd["rollout_length"]), float(cmd["horizon_length"]), dt)
+        b = gen.param_bounds()
+        P = torch.tensor(rng.uniform(b[:, 0], b[:, 1], size=(n_trials, len(b))), dtype=torch.float32,
+                         device="cuda")
+        ms = []
+        for rep in range(warmup + steps):
+            a0, a1 = _events(stream)
+            a0.record(stream)
+            out = gen.generate(P)
+            a1.record(stream)
+            torch.cuda.synchronize()
+            if rep >= warmup:
+                ms.append(a0.elapsed_time(a1))
+        t = float(np.mean(ms)) / 1e3
+        nbytes = 4.0 * out.numel() + 4.0 * P.numel()
+        modes[mode] = {"ms": round(t * 1e3, 4), "trials_per_s": round(n_trials / t, 1),
+                       "GB_per_s": round(nbytes / t / 1e9, 1), "frac_hbm": round(nbytes / t / 1e9 / HBM_PEAK_GBS, 4),
+                       "params_per_trial": int(P.shape[1])}
+    worst = max(modes.values(), key=lambda m: m["ms"])
+    return {"metric": "command sequences/s", "value": worst["trials_per_s"], "unit": "trials/s (slowest mode)",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": worst["ms"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+            "config": {"workload": f"commands_{n_trials}x{gen.total_steps}x{gen.n_dims}", "trials": n_trials,
+                       "steps": gen.total_steps, "dims": gen.n_dims, "sampled": list(gen.sampled)},
+            "modes": modes}
