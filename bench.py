@@ -56,10 +56,10 @@ def parse():
                    help="Levenberg-Marquardt iterations after CEM, inside each timed solve (0: CEM only)")
     p.add_argument("--workload", default="cem22",
                    choices=["cem22", "landscape", "batch64", "fisher", "gradient", "serve", "store", "commands",
-                            "policy"],
+                            "study", "policy"],
                    help="cem22 (default): the 22-parameter CEM solve; the others are the plain-method landscape, "
                         "64 batched uneven problems, Fisher scoring of excitations, the gradient, the serving "
-                        "loop, dataset windows, command sequences and the policy-in-the-loop rollout")
+                        "loop, dataset windows, command sequences, the ask/tell study and the policy-in-the-loop rollout")
     p.add_argument("--requests", type=int, default=10000, help="serve: number of requests")
     p.add_argument("--policy-batches", default="1024,8192",
                    help="policy: trials per iteration at the reference batch[,wide batch]")

@@ -78,6 +78,7 @@ class DeviceStudy:
         self._batch_rows: Optional[np.ndarray] = None
         self._batch: List[DeviceTrial] = []
         self._told: Dict[int, float] = {}
+        self._n_told = 0
         self._iteration = 0
         self._best: Optional[FrozenTrial] = None
 
@@ -101,8 +102,11 @@ class DeviceStudy:
         key = v if self.direction == "minimize" else -v
         if self._best is None or key < (self._best.value if self.direction == "minimize" else -self._best.value):
             self._best = self.trials[trial.number]
+        if trial.number not in self._told and self._batch and \
+                self._batch[0].number <= trial.number <= self._batch[-1].number:
+            self._n_told += 1                        # trials of the current batch told so far
         self._told[trial.number] = v
-        if len(self._batch) == self.sampler.batch_size and all(t.number in self._told for t in self._batch):
+        if len(self._batch) == self.sampler.batch_size and self._n_told == len(self._batch):
             self._update()
 
     @property
@@ -142,7 +146,7 @@ class DeviceStudy:
                                        _lib.stream())
         _lib.check(rc, "spi_cem_sample")
         self._batch_rows = self.theta.t().cpu().numpy().astype(np.float64)
-        self._batch, self._told = [], {}
+        self._batch, self._told, self._n_told = [], {}, 0
 
     def _update(self):
         B = self.sampler.batch_size
@@ -185,7 +189,7 @@ class DeviceStudy:
             self._init_cem()
             self.mean.copy_(torch.from_numpy(np.asarray(d["mean"], np.float32)).to(self.device))
             self.std.copy_(torch.from_numpy(np.asarray(d["std"], np.float32)).to(self.device))
-        self._batch, self._told, self._batch_rows = [], {}, None
+        self._batch, self._told, self._batch_rows, self._n_told = [], {}, None, 0
 
 
 def create_study(direction: str = "minimize", sampler: Optional[DeviceCEMSampler] = None,

@@ -12,6 +12,7 @@
   continuous-batching server under an HBM budget.
 * ``store`` -- dataset windows: 64 trajectories x 2500 steps packed into H 20 segment records.
 * ``commands`` -- command sequences of one 1024-trial ask batch in the three sampling modes.
+* ``study`` -- the Optuna-shaped ask / suggest_float / tell loop, 5 iterations of 1024 trials.
 
 Each returns the dict ``bench.py`` prints as its JSON line.
 """
@@ -542,3 +543,40 @@ def commands(steps: int, warmup: int, n_trials: int = 1024) -> Dict:
             "config": {"workload": f"commands_{n_trials}x{gen.total_steps}x{gen.n_dims}", "trials": n_trials,
                        "steps": gen.total_steps, "dims": gen.n_dims, "sampled": list(gen.sampled)},
             "modes": modes}
+
+
+def study(steps: int, warmup: int, batch: int = 1024, iterations: int = 5) -> Dict:
+    """The Optuna-shaped study (active_sysid.py:185-217's ask / suggest_float / tell loop) over
+    the device CEM at the reference's shape: 5 iterations of 1024 trials, each suggesting the 15
+    floats of the constant command mode (3 sampled dimensions x 5 updates) and told a value.
+    The value is a host-side quadratic in the suggestions (the policy rollout that would
+    produce it is measured by ``--workload policy``), so the figure is the shim's own cost: the
+    per-trial Python surface plus one sample and one update launch per iteration."""
+    from .agents.study import DeviceCEMSampler, create_study
+    names = [f"dim_{a}_update_{u}" for a in (0, 2, 5) for u in range(5)]
+    target = np.random.default_rng(0).uniform(-0.5, 0.5, len(names))
+    per_iter, best = [], []
+    for rep in range(warmup + steps):
+        st = create_study(direction="maximize", sampler=DeviceCEMSampler(batch_size=batch, seed=rep))
+        torch.cuda.synchronize()
+        times = []
+        for it in range(iterations):
+            t0 = time.perf_counter()
+            trials = [st.ask() for _ in range(batch)]
+            for t in trials:
+                x = np.array([t.suggest_float(n, -1.0, 1.0) for n in names])
+                st.tell(t, -float(((x - target) ** 2).sum()))
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        if rep >= warmup:
+            per_iter.append(times)
+            best.append(st.best_value)
+    it_ms = float(np.mean(per_iter)) * 1e3
+    return {"metric": "trials/s through ask, suggest_float and tell", "value": round(batch / (it_ms / 1e3), 1),
+            "unit": "trials/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(it_ms, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (CEM), host floats",
+            "data": "synthetic",
+            "config": {"workload": f"study_{batch}x{iterations}_15params", "trials_per_iteration": batch,
+                       "iterations": iterations, "params": len(names)},
+            "ms_per_iteration_by_index": [round(float(x) * 1e3, 2) for x in np.mean(per_iter, axis=0)],
+            "best_value_after_5_iterations": float(np.mean(best))}
