@@ -38,9 +38,11 @@ __global__ __launch_bounds__(256) void pack_windows_kernel(const float* __restri
   const int nw = min(kWinPerBlock, n_windows - w0);
   const int r0 = rows ? rows[w0] : first + w0;
   const int j = threadIdx.x;
-  const int rj = j < nw ? (rows ? rows[w0 + j] : first + w0 + j) : r0 + j;
-  const bool staged = H <= kMaxStagedH && __syncthreads_and(rj == r0 + j) && r0 >= 0 &&
-                      r0 + nw - 1 + H < n_rows;
+  // unsigned: "window j starts j rows after window 0" without signed overflow
+  const unsigned rj = j < nw ? static_cast<unsigned>(rows ? rows[w0 + j] : first + w0 + j)
+                             : static_cast<unsigned>(r0) + j;
+  const bool staged = H <= kMaxStagedH && __syncthreads_and(rj == static_cast<unsigned>(r0) + j) && r0 >= 0 &&
+                      static_cast<long long>(r0) + nw - 1 + H < n_rows;
   float* o = out + static_cast<size_t>(w0) * stride;
   if (staged) {
     const int n_stage = (nw + H) * kRow;
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(256) void pack_windows_kernel(const float* __restri
   auto ac = [&](int row, int c) { return actions[static_cast<size_t>(row) * 12 + c]; };
   for (int w = 0; w < nw; ++w) {
     const int t = rows ? rows[w0 + w] : first + w0 + w;
-    const bool ok = t >= 0 && t + H < n_rows;
+    const bool ok = t >= 0 && static_cast<long long>(t) + H < n_rows;
     for (int e = threadIdx.x; e < stride; e += blockDim.x)
       o[static_cast<size_t>(w) * stride + e] = ok ? record_elem(e, t, st, ac) : __int_as_float(0x7fc00000);
   }
@@ -88,7 +90,8 @@ extern "C" {
 
 int spi_pack_windows(const float* states_dev, const float* actions_dev, int n_steps, int horizon, int first,
                      int n_windows, float* segments_dev, hipStream_t stream) {
-  if (n_steps < 0 || horizon < 1 || first < 0 || n_windows < 0 || first + n_windows + horizon > n_steps ||
+  if (n_steps < 0 || horizon < 1 || first < 0 || n_windows < 0 ||
+      static_cast<long long>(first) + n_windows + horizon > n_steps ||
       (n_windows > 0 && (!states_dev || !actions_dev || !segments_dev))) {
     spi_set_error("spi_pack_windows: windows must lie inside the trajectory (first + n + horizon <= n_steps)");
     return SPI_ERR_INVALID_ARG;
