@@ -189,10 +189,12 @@ def test_cem_and_refinement_find_fp64_oracle_minimiser(gpu, model, sim):
     assert err.max() <= 2e-2, {PARAM_NAMES[rows[k]]: float(err[k]) for k in range(len(rows)) if err[k] > 2e-2}
 
 
-def test_speculative_jacobians_leave_the_iterates_unchanged(gpu, sim, theta0):
+@pytest.mark.parametrize("legs", [True, False])
+def test_speculative_jacobians_leave_the_iterates_unchanged(gpu, sim, theta0, legs):
     """Jacobians started early at the full and half step (beside the line search on a second
     stream) give bit for bit the iterates of the plain loop, whichever line-search point wins,
-    and a rejected step followed by a retry reuses the current point's Jacobian."""
+    and a rejected step followed by a retry reuses the current point's Jacobian; with four
+    lanes per tangent and with one (two candidates per Jacobian launch in both)."""
     from spi_amd.agents.refine import GaussNewtonRefiner
     from spi_amd.agents.sysid import identification_prior
     from spi_amd.rollout import RolloutEngine
@@ -205,7 +207,7 @@ def test_speculative_jacobians_leave_the_iterates_unchanged(gpu, sim, theta0):
     start[:22] *= 1 + 0.03 * rng.normal(size=22)
     runs = {}
     for spec in (False, True):
-        ref = GaussNewtonRefiner(eng, seg_dev, 10, lo=prior.lo, hi=prior.hi)
+        ref = GaussNewtonRefiner(eng, seg_dev, 10, lo=prior.lo, hi=prior.hi, legs=legs)
         runs[spec] = ref.refine(start, iterations=12, speculate=spec)
     a, b = runs[False], runs[True]
     print("taken:", b.taken, "accepted", b.accepted, "of", b.iterations, "history", b.history)
