@@ -70,15 +70,24 @@ template <int kC> __device__ __forceinline__ int lds_at_c(int row, int k) {
 // For small batches, whose 32 x 128 tiles leave SIMDs idle or alone: kS wavefronts per SIMD
 // hide each other's load latency and each K loop is kS times shorter.  kS = 1 is the plain
 // tile loop.
-template <bool kElu, int kSrc, int kS>
+// kFuse: the layer's outputs (N <= 128: one column tile) never leave the workgroup; after the
+// epilogue (bias, ELU) they go to LDS and the workgroup computes the next, last layer from them
+// (W2 [N2][N], b2, N2 <= 32, no activation: for each row and output, multiply-add chains over
+// the two halves of the N inputs in index order, then the two halves and the bias in that
+// order), writing Y2 [M][N2].
+template <bool kElu, int kSrc, int kS, bool kFuse = false>
 __global__ __launch_bounds__(256 * kS) void linear_kernel(const float* __restrict__ X, int ldx, RingSrc rs, int M, int K,
                                                           const float* __restrict__ W, const float* __restrict__ b, int N,
-                                                          float* __restrict__ Y, int ldy) {
+                                                          float* __restrict__ Y, int ldy,
+                                                          const float* __restrict__ W2 = nullptr,
+                                                          const float* __restrict__ b2 = nullptr, int N2 = 0,
+                                                          float* __restrict__ Y2 = nullptr) {
   constexpr int kRows = 32;
   constexpr int kKs = kS == 4 ? 32 : kKc;              // K chunk of one group
   constexpr int kPs = kKs + 1;
   constexpr int kGroupFloats = (kRows + kCols) * kPs;
   static_assert(kS == 1 || (kS - 1) * 16 * 256 <= kS * kGroupFloats, "reduction buffer fits the tiles");
+  static_assert(kRows * (kCols + 1) + 2 * kRows * 32 <= kS * kGroupFloats, "a fused tail's tile fits the tiles");
   __shared__ float smem[kS * kGroupFloats];
   __shared__ int Kr[kSrc >= 2 ? kMaxRingK : 1];       // ring line of each input column
   const int m0 = blockIdx.x * kRows, n0 = blockIdx.y * kCols;
@@ -198,12 +207,53 @@ __global__ __launch_bounds__(256 * kS) void linear_kernel(const float* __restric
       for (int r = 0; r < 16; ++r) red[((grp - 1) * 16 + r) * 256 + tid] = acc[r];
     }
     __syncthreads();
-    if (grp > 0) return;
-    if (live) {
+    if (grp > 0 && !kFuse) return;
+    if (grp == 0 && live) {
       for (int g = 1; g < kS; ++g)
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] += red[((g - 1) * 16 + r) * 256 + tid];
     }
+  }
+  if (kFuse) {
+    // every thread of every group reaches both barriers; group 0 holds the finished tile
+    constexpr int kHs = kCols + 1;
+    float* h = smem;                                   // [32][kHs] after the reduction reads
+    __syncthreads();
+    if (grp == 0) {
+      const int gn = wave * 32 + col;
+      const float bias = gn < N ? b[gn] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * half + (r & 3);
+        float v = 0.0f;
+        if (live && gn < N) {
+          v = acc[r] + bias;
+          if (kElu) v = v > 0.0f ? v : expm1f(v);
+        }
+        h[row * kHs + gn] = v;
+      }
+    }
+    __syncthreads();
+    // every thread: partial sums over the two halves of the N inputs, (half, row, output) per
+    // thread, then half 0 + half 1 + bias in that order
+    float* part = h + kRows * kHs;                     // [2][32][32]
+    const int nh = (N + 1) / 2;
+    for (int u = threadIdx.x; u < 2 * kRows * N2; u += blockDim.x) {
+      const int q = u / (kRows * N2), rem = u - q * (kRows * N2), row = rem / N2, j = rem - row * N2;
+      const int k0 = q * nh, k1 = min(N, k0 + nh);
+      const float* hr = h + row * kHs;
+      const float* w2 = W2 + static_cast<size_t>(j) * N;
+      float sum = 0.0f;
+#pragma unroll 8
+      for (int k = k0; k < k1; ++k) sum = fmaf(hr[k], w2[k], sum);
+      part[(q * kRows + row) * 32 + j] = sum;
+    }
+    __syncthreads();
+    for (int u = threadIdx.x; u < kRows * N2; u += blockDim.x) {
+      const int row = u / N2, j = u - row * N2, gm = m0 + row;
+      if (gm < M) Y2[static_cast<size_t>(gm) * N2 + j] = (part[row * 32 + j] + part[(kRows + row) * 32 + j]) + b2[j];
+    }
+    return;
   }
   if (!live) return;
   const int gn = n0 + wave * 32 + col;
@@ -450,6 +500,26 @@ int launch_linear(const float* X, int ldx, const RingSrc& rs, int M, int K, cons
   return SPI_ERR_HIP;
 }
 
+// The last two layers in one launch (linear_kernel with kFuse) when the second-to-last has at
+// most 128 outputs (one column tile) and the last at most 32, and the input is a dense buffer
+// with 16-byte rows; returns false (nothing launched) when that does not hold.
+bool fused_tail(const float* X, int ldx, int M, int K, const float* W, const float* b, int N, const float* W2,
+                const float* b2, int N2, float* Y2, hipStream_t stream, int* rc) {
+  if (N > kCols || N2 > 32 || K % 4 != 0 || ldx % 4 != 0 || !aligned16(W) || !aligned16(X)) return false;
+  const RingSrc none{nullptr, 0, 0, 0, 0, nullptr, 0.0f};
+  const dim3 grid((M + 31) / 32, 1);
+  const int split = split_for(M, K, N);
+  if (split == 4)
+    linear_kernel<true, 1, 4, true><<<grid, 1024, 0, stream>>>(X, ldx, none, M, K, W, b, N, nullptr, 0, W2, b2, N2, Y2);
+  else if (split == 2)
+    linear_kernel<true, 1, 2, true><<<grid, 512, 0, stream>>>(X, ldx, none, M, K, W, b, N, nullptr, 0, W2, b2, N2, Y2);
+  else
+    linear_kernel<true, 1, 1, true><<<grid, 256, 0, stream>>>(X, ldx, none, M, K, W, b, N, nullptr, 0, W2, b2, N2, Y2);
+  const hipError_t e = hipGetLastError();
+  *rc = e == hipSuccess ? SPI_OK : mfail(SPI_ERR_HIP, hipGetErrorString(e));
+  return true;
+}
+
 long long param_floats(int n_layers, const int* dims) {
   if (n_layers < 1 || n_layers > kMaxLayers || !dims) return -1;
   long long n = 0;
@@ -481,6 +551,14 @@ int mlp_layers(const float* x_dev, const RingSrc& rs, int M, int n_layers, const
   for (int l = 0; l < n_layers; ++l) {
     const int K = dims[l], N = dims[l + 1];
     const bool last = l == n_layers - 1;
+    if (l > 0 && l == n_layers - 2) {                 // hidden layer + output layer in one launch
+      const float* p2 = p + static_cast<size_t>(N) * K + N;
+      const int N2 = dims[l + 2];
+      int rc = SPI_OK;
+      if (fused_tail(in, K, M, K, p, p + static_cast<size_t>(N) * K, N, p2, p2 + static_cast<size_t>(N2) * N, N2,
+                     y_dev, stream, &rc))
+        return rc;
+    }
     float* out = last ? y_dev : work_dev + static_cast<size_t>(l & 1) * M * widest;
     const RingSrc none{nullptr, 0, 0, 0, 0, nullptr, 0.0f};
     const int rc = launch_linear(in, K, l == 0 ? rs : none, M, K, p, p + static_cast<size_t>(N) * K, N, out, N,

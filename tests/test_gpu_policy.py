@@ -42,11 +42,16 @@ def _commands(rng, M, T):
 
 @pytest.mark.parametrize("M,dims", [(1000, [900, 512, 256, 128, 12]), (37, [70, 33, 12]), (5, [64, 1]),
                                     (9001, [900, 512, 256, 128, 12]), (8200, [70, 33, 12]),
-                                    (2048, [900, 512, 256, 128, 12]), (600, [130, 257, 12])])
+                                    (2048, [900, 512, 256, 128, 12]), (600, [130, 257, 12]),
+                                    (16400, [64, 96, 128, 12]), (517, [40, 64, 96, 20]),
+                                    (300, [64, 200, 160, 12])])
 def test_mlp_matches_float64(gpu, M, dims):
     """MFMA FP32 layers (ELU between) vs float64 torch on the CPU, ragged tile edges included;
     the shapes cover the K split in 1, 2 and 4 groups (2048 rows: layer 1 in 2 groups with a
-    ragged last chunk; 1000 and 600 rows: 4 groups, 600 with scalar loads for odd K)."""
+    ragged last chunk; 1000 and 600 rows: 4 groups, 600 with scalar loads for odd K) and the
+    last two layers in one launch (hidden width <= 128, output <= 32: the policy's 128 -> 12 with
+    the K split in 4 (1000, 2048 rows), 2 (9001) and 1 group (16400); 96 hidden units leave a
+    wavefront without columns (517 rows)); 160 hidden units take two launches (300 rows)."""
     from spi_amd.agents.policy import PolicyMLP
     net = PolicyMLP.random(dims[0], dims[-1], hidden=dims[1:-1], seed=3)
     x = torch.randn(M, dims[0], generator=torch.Generator().manual_seed(1)) * 2
