@@ -273,7 +273,8 @@ def main():
             ls = workloads.landscape(steps=200, warmup=5)
             extras["landscape_mass_256x64_h5"] = {"solve_latency_ms": ls["solve_latency_ms"],
                                                   "solves_per_s": ls["solves_per_s"],
-                                                  "argmin_mass_kg": ls["argmin_mass_kg"]}
+                                                  "argmin_mass_kg": ls["argmin_mass_kg"],
+                                                  "cpu_baseline": ls["cpu_baseline"]}
         except Exception as e:  # reported, never fatal
             extras["landscape_mass_256x64_h5"] = {"failed": str(e)}
         try:

@@ -45,6 +45,30 @@ def _settled_state(eng: RolloutEngine, theta: np.ndarray, steps: int = 50) -> np
     return s[0, -1]
 
 
+def _host_cores() -> int:
+    """Cores this process may use: OMP_NUM_THREADS when set, else the affinity mask."""
+    import os
+    env = os.environ.get("OMP_NUM_THREADS", "").strip()
+    return int(env) if env.isdigit() and int(env) > 0 else len(os.sched_getaffinity(0))
+
+
+def _cpu_rollout_lib():
+    """The rollout kernels' algorithm built for the host (spi_dyn.h, x86-64-v3, OpenMP:
+    tests/host_dyn/spi_rollout_cpu.cpp), the same-algorithm CPU baseline."""
+    import ctypes
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    path = os.path.join(root, "tests", "host_dyn", "lib", "libspi_rollout_cpu.so")
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", os.path.join(root, "tests", "host_dyn"), "-s"], check=True)
+    L = ctypes.CDLL(path)
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.spi_cpu_rollout_cost.argtypes = [ctypes.c_void_p, ctypes.c_void_p, fp, ctypes.c_int, ctypes.c_int, fp,
+                                       ctypes.c_int, ctypes.c_int, fp, ctypes.c_int]
+    return L
+
+
 # ------------------------------------------------------------------ plain method (1-D landscape)
 def landscape(steps: int, warmup: int) -> Dict:
     sim = SimParams()
@@ -92,6 +116,30 @@ def landscape(steps: int, warmup: int) -> Dict:
     ms = np.array([a.elapsed_time(b) for a, b in lat])
     idx = int(idx_dev.item())
     ctrl = 256 * 64 * 5
+    # the same solve by the same algorithm on the host cores (median of 3), and its argmin
+    try:
+        import ctypes
+        from . import _lib
+        L = _cpu_rollout_lib()
+        fp = ctypes.POINTER(ctypes.c_float)
+        a_h, c_h = _lib.sim_args(sim), _lib.cost_args(eng.cost_mode, eng.weights)
+        seg = np.ascontiguousarray(segs.packed(), np.float32)
+        par = np.ascontiguousarray(rows.T.astype(np.float32))
+        out = np.zeros((64, 256), np.float32)
+        cores, times = _host_cores(), []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            used = L.spi_cpu_rollout_cost(ctypes.byref(a_h), ctypes.byref(c_h), par.ctypes.data_as(fp), 256, 256,
+                                          seg.ctypes.data_as(fp), 64, 5, out.ctypes.data_as(fp), cores)
+            times.append(time.perf_counter() - t0)
+        cpu_ms = float(np.median(times)) * 1e3
+        cpu = {"kind": "port", "ms_per_solve": round(cpu_ms, 3), "cores": int(used),
+               "solves_per_s": round(1e3 / cpu_ms, 2),
+               "argmin_mass_kg": float(masses[int(np.argmin(out.astype(np.float64).sum(axis=0)))]),
+               "gpu_speedup_p50": round(cpu_ms / float(np.percentile(ms, 50)), 1),
+               "sample": "the whole solve (256 masses x 64 segments x H 5), median of 3"}
+    except Exception as e:  # reported, never fatal
+        cpu = {"value": None, "failed": str(e)}
     return {"metric": "env-steps/s, landscape solves/s; p50/p99 solve latency", "value": round(steps * ctrl / total, 1),
             "unit": "env-steps/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": round(total / steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -102,7 +150,7 @@ def landscape(steps: int, warmup: int) -> Dict:
             "solve_latency_ms": {"p50": round(float(np.percentile(ms, 50)), 4),
                                  "p99": round(float(np.percentile(ms, 99)), 4)},
             "argmin_mass_kg": float(masses[idx]), "truth_kg": float(truth[P_MASS]),
-            "grid_step_kg": float(masses[1] - masses[0])}
+            "grid_step_kg": float(masses[1] - masses[0]), "cpu_baseline": cpu}
 
 
 # ------------------------------------------------------------------ batched uneven problems
